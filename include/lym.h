@@ -1,0 +1,114 @@
+/*
+ * lym.h -- C ABI of the MI355X (gfx950) hot path of LatticeYangMills.
+ *
+ * The reference has no FFI: its boundary is the C++ virtual class API of
+ * Action / Observable / App (include/Actions/action.h:54-68,
+ * include/Observables/observable.h:52-66, include/Apps/app.h:59-77).
+ * Each entry point below replaces one member function of that API; the C++
+ * host classes in latticeyangmills_amd/host call these.
+ *
+ * Conventions
+ *   - Link data crossing the boundary is the reference's GluonField layout:
+ *     four arrays (one per direction mu), each V x 18 doubles, site index
+ *     s = x + Lx*(y + Ly*(z + Lz*t)), element m[2*(3i+j)] = Re U_ij,
+ *     m[2*(3i+j)+1] = Im U_ij (include/Math/field.h:52-76,
+ *     include/Math/lattice.h:87-89, include/Math/su3.h:63).
+ *   - Every function returns LYM_OK (0) or a negative LYM_ERR_* code and
+ *     never throws; lym_last_error() gives the message of the last failure.
+ *   - A context owns its device buffers and one HIP stream; calls on one
+ *     context are not thread-safe, distinct contexts are independent.
+ */
+#ifndef LYM_H
+#define LYM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYM_OK 0
+#define LYM_ERR_ARG (-1)    /* invalid argument */
+#define LYM_ERR_HIP (-2)    /* HIP runtime error */
+#define LYM_ERR_NOMEM (-3)  /* device allocation failed */
+#define LYM_ERR_NODEV (-4)  /* no GPU visible */
+
+#define LYM_PREC_FP64 0  /* FP64 links, FP64 arithmetic throughout */
+#define LYM_PREC_MIXED 1 /* FP32 links/staples/clover; FP64 exp, reunitarisation, reductions */
+
+#define LYM_CAND_SU2 0  /* X = R*S*T of SU(2) subgroups, Random::randSU3Transf (src/Math/random.cpp:71-156) */
+#define LYM_CAND_EXPH 1 /* X = exp(i eps H), H = sum_a h_a lambda_a, h_a uniform in [-1, 1) */
+
+typedef struct lym_ctx lym_ctx;
+
+/* Creates a context for one Lx*Ly*Lz*Lt periodic lattice on GPU `device`;
+ * links start cold (all 1).  Replaces WilsonFlow::createLattice /
+ * App::createLattice (src/Apps/wilsonflow.cpp:87-91, src/Apps/app.cpp:41-44). */
+int lym_create(lym_ctx **ctx, const int dims[4], int precision, int device);
+int lym_destroy(lym_ctx *ctx);
+const char *lym_last_error(const lym_ctx *ctx);
+/* V = number of sites; precision as given to lym_create. */
+int lym_info(const lym_ctx *ctx, int dims[4], int64_t *volume, int *precision);
+
+/* Host <-> device copies of the gauge field, reference layout (see top).
+ * Upload replaces InputConf::readConf's fill of the GluonField
+ * (src/InputOutput/inputconf.cpp:114-155). */
+int lym_upload_links(lym_ctx *ctx, const double *const by_mu[4]);
+int lym_download_links(lym_ctx *ctx, double *const by_mu[4]);
+/* Cold start, all U = 1 (setToIdentity, src/Math/lattice.cpp:33-41). */
+int lym_set_cold(lym_ctx *ctx);
+/* Hot start from Haar-random SU(3) (the reference's GaugeFieldFactory "Hot"
+ * start, src/Apps/gaugefieldfactory.cpp:67-71, whose setToRandom is missing;
+ * Philox-seeded here). */
+int lym_set_hot(lym_ctx *ctx, uint64_t seed);
+
+/* PureGauge::computeStaples(mu) (src/Actions/puregauge.cpp:36-54):
+ * omega_out = V x 18 doubles. */
+int lym_staples(lym_ctx *ctx, int mu, double *omega_out);
+/* PureGauge::computeDerivative(mu) (src/Actions/puregauge.cpp:62-79):
+ * z_out = V x 18 doubles, Z = (i/2)[U Omega - (U Omega)^+]_traceless. */
+int lym_derivative(lym_ctx *ctx, int mu, double *z_out);
+
+/* SuperObs::compute (src/Observables/superobs.cpp:31-128):
+ * out[0] = plaq, out[1] = energy (raw sum of Re Tr C_{mu nu}^2, <= 0),
+ * out[2] = topc, out[3] = sum over sites of the absolute values of the three
+ * plane-pair terms of q(x), normalised like topc (the scale of the cancellations
+ * in topc). */
+int lym_observables(lym_ctx *ctx, double out[4]);
+
+/* nsteps x WilsonFlow::flowStep (src/Apps/wilsonflow.cpp:141-172); enqueued
+ * on the context's stream, returns before completion. */
+int lym_flow_steps(lym_ctx *ctx, double eps, int nsteps);
+/* WilsonFlow::applyWilsonFlow (src/Apps/wilsonflow.cpp:115-136): rows =
+ * int(tau/eps) series rows, rows-1 flow steps; row t = [t, plaq, topc, energy]
+ * with t accumulated by += eps.  Rows with t % measure_every != 0 hold NaN
+ * observables (measure_every = 1 is the reference).  series holds max_rows*4
+ * doubles; *rows_out receives rows. */
+int lym_flow(lym_ctx *ctx, double eps, double tau, int measure_every, double *series, int max_rows,
+             int *rows_out);
+
+/* GaugeFieldFactory::MCUpdate / updateLink (src/Apps/gaugefieldfactory.cpp:192-220)
+ * with the Delta S of PureGauge::compute (src/Actions/puregauge.cpp:28), as a
+ * checkerboard: per sweep, sub-sweeps mu = 0..3, parity 0 then 1.  Sweeps are
+ * numbered first_sweep .. first_sweep+nsweeps-1 (Philox counter).  predrawn ==
+ * NULL: Philox-4x32-10 uniforms, counter {mu*V+s, sweep, hit, k/4}, key seed.
+ * predrawn != NULL (test mode): uniform k of hit h of link l in sweep w is
+ * predrawn[((w*4V + l)*nhit + h)*n + k], n = 13 (LYM_CAND_SU2) or 9 (LYM_CAND_EXPH).
+ * Extents must be even.  *accepted = number of accepted hits. */
+int lym_metropolis(lym_ctx *ctx, double beta, double eps, int nhit, int nsweeps, int first_sweep,
+                   int candidate, uint64_t seed, const double *predrawn, int64_t *accepted);
+
+/* Blocks until all work on the context's stream is done. */
+int lym_synchronize(lym_ctx *ctx);
+
+/* Measurement helper: runs nsteps flow steps with a HIP event pair around
+ * every kernel launch on the context's stream.  stage_ms[k] = mean duration
+ * of RK stage k+1 (k = 0..2), stage_ms[3] = mean duration of the observables
+ * kernel (measured every measure_every steps, 0 = never); *total_ms = wall
+ * time of the whole sequence on the stream. */
+int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, double *total_ms,
+                     double stage_ms[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,471 @@
+// C ABI of the single-lattice path (declared in include/lym.h).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lym.h"
+#include "lym_kernels.hpp"
+
+using lym::Geom;
+using lym::ksum;
+
+struct lym_ctx {
+    int device = 0;
+    int prec = LYM_PREC_FP64;
+    int dims[4] = {0, 0, 0, 0};
+    Geom g{};
+    hipStream_t stream = nullptr;
+    void *W[2] = {nullptr, nullptr};  // double-buffered link field (SoA)
+    int cur = 0;                      // W[cur] holds the current links
+    void *acc = nullptr;              // RK accumulator, 8 reals per link
+    ksum *partials = nullptr;         // per-workgroup observables partials
+    int nblk_obs = 0;
+    double *obs = nullptr;  // measurement slots, 4 doubles each
+    int obs_cap = 0;
+    double *staging = nullptr;  // V x 18 doubles (reference layout of one direction)
+    unsigned long long *accepted = nullptr;
+    double *predrawn = nullptr;
+    size_t predrawn_cap = 0;
+    std::vector<hipEvent_t> events;
+    std::string err;
+};
+
+namespace {
+
+int fail(lym_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(lym_ctx *c, hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(c, e == hipErrorOutOfMemory ? LYM_ERR_NOMEM : LYM_ERR_HIP, m);
+}
+
+#define LYM_HIP(c, call)                                   \
+    do {                                                   \
+        hipError_t e_ = (call);                            \
+        if (e_ != hipSuccess) return hipfail(c, e_, #call); \
+    } while (0)
+
+#define LYM_LAUNCHED(c, what)                              \
+    do {                                                   \
+        hipError_t e_ = hipGetLastError();                 \
+        if (e_ != hipSuccess) return hipfail(c, e_, what); \
+    } while (0)
+
+size_t real_size(const lym_ctx *c) { return c->prec == LYM_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+unsigned blocks_for(long n, int per) { return unsigned((n + per - 1) / per); }
+
+template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
+{
+    using F2 = lym::F2<S>;
+    F2 *a = static_cast<F2 *>(c->W[c->cur]);
+    F2 *b = static_cast<F2 *>(c->W[1 - c->cur]);
+    S *acc = static_cast<S *>(c->acc);
+    const unsigned nb = blocks_for(c->g.V, lym::kSitesPerBlock);
+    constexpr bool reunit = sizeof(S) == 4;
+    if (ev) hipEventRecord(ev[0], c->stream);
+    lym::k_flow_stage<S, 1, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, eps);
+    if (ev) hipEventRecord(ev[1], c->stream);
+    lym::k_flow_stage<S, 2, reunit><<<nb, 256, 0, c->stream>>>(b, a, acc, c->g, eps);
+    if (ev) hipEventRecord(ev[2], c->stream);
+    lym::k_flow_stage<S, 3, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, eps);
+    if (ev) hipEventRecord(ev[3], c->stream);
+    LYM_LAUNCHED(c, "k_flow_stage");
+    c->cur = 1 - c->cur;
+    return LYM_OK;
+}
+
+int step(lym_ctx *c, double eps, hipEvent_t *ev = nullptr)
+{
+    return c->prec == LYM_PREC_FP64 ? enqueue_step<double>(c, eps, ev) : enqueue_step<float>(c, eps, ev);
+}
+
+// observables of the current field into measurement slot `slot`
+template <typename S> int enqueue_observables(lym_ctx *c, int slot, hipEvent_t *ev)
+{
+    const lym::F2<S> *f = static_cast<const lym::F2<S> *>(c->W[c->cur]);
+    if (ev) hipEventRecord(ev[0], c->stream);
+    lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, c->partials);
+    if (ev) hipEventRecord(ev[1], c->stream);
+    lym::k_reduce_partials<<<1, 256, 0, c->stream>>>(c->partials, c->nblk_obs, c->obs + 4 * slot);
+    LYM_LAUNCHED(c, "k_observables");
+    return LYM_OK;
+}
+
+int observables(lym_ctx *c, int slot, hipEvent_t *ev = nullptr)
+{
+    return c->prec == LYM_PREC_FP64 ? enqueue_observables<double>(c, slot, ev)
+                                    : enqueue_observables<float>(c, slot, ev);
+}
+
+int ensure_slots(lym_ctx *c, int n)
+{
+    if (n <= c->obs_cap) return LYM_OK;
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->obs) hipFree(c->obs);
+    c->obs = nullptr;
+    LYM_HIP(c, hipMalloc(&c->obs, sizeof(double) * 4 * size_t(n)));
+    c->obs_cap = n;
+    return LYM_OK;
+}
+
+// raw sums -> reference normalisation (src/Observables/superobs.cpp:125-126)
+void normalise(const lym_ctx *c, const double *raw, double out[4])
+{
+    const double qn = 16 * 16 * M_PI * M_PI;
+    out[0] = raw[0] / 18.0 / double(c->g.V);
+    out[1] = raw[1];
+    out[2] = raw[2] / qn;
+    out[3] = raw[3] / qn;
+}
+
+int ensure_events(lym_ctx *c, size_t n)
+{
+    while (c->events.size() < n) {
+        hipEvent_t e;
+        LYM_HIP(c, hipEventCreate(&e));
+        c->events.push_back(e);
+    }
+    return LYM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
+{
+    if (!out || !dims) return LYM_ERR_ARG;
+    *out = nullptr;
+    if (precision != LYM_PREC_FP64 && precision != LYM_PREC_MIXED) return LYM_ERR_ARG;
+    long V = 1;
+    for (int d = 0; d < 4; d++) {
+        if (dims[d] < 2) return LYM_ERR_ARG;
+        V *= dims[d];
+    }
+    if (4 * V >= (1L << 31)) return LYM_ERR_ARG;  // link ids are 32-bit Philox counters
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
+    if (device < 0 || device >= ndev) return LYM_ERR_ARG;
+    lym_ctx *c = new lym_ctx;
+    c->device = device;
+    c->prec = precision;
+    long st = 1;
+    for (int d = 0; d < 4; d++) {
+        c->dims[d] = dims[d];
+        c->g.L[d] = dims[d];
+        c->g.stride[d] = int(st);
+        st *= dims[d];
+    }
+    c->g.V = V;
+    c->nblk_obs = int(blocks_for(V, lym::kObsSites));
+    auto bail = [&](int code) {
+        lym_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(LYM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(LYM_ERR_HIP);
+    const size_t rs = real_size(c);
+    for (int k = 0; k < 2; k++)
+        if (hipMalloc(&c->W[k], rs * 2 * 9 * 4 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&c->acc, rs * 8 * 4 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&c->partials, sizeof(ksum) * 4 * size_t(c->nblk_obs)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&c->staging, sizeof(double) * 18 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
+    int rc = lym_set_cold(c);
+    if (rc != LYM_OK) return bail(rc);
+    *out = c;
+    return LYM_OK;
+}
+
+int lym_destroy(lym_ctx *c)
+{
+    if (!c) return LYM_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
+                    (void *)c->accepted, (void *)c->predrawn})
+        if (p) hipFree(p);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return LYM_OK;
+}
+
+const char *lym_last_error(const lym_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int lym_info(const lym_ctx *c, int dims[4], int64_t *volume, int *precision)
+{
+    if (!c) return LYM_ERR_ARG;
+    if (dims)
+        for (int d = 0; d < 4; d++) dims[d] = c->dims[d];
+    if (volume) *volume = c->g.V;
+    if (precision) *precision = c->prec;
+    return LYM_OK;
+}
+
+int lym_upload_links(lym_ctx *c, const double *const by_mu[4])
+{
+    if (!c || !by_mu) return LYM_ERR_ARG;
+    for (int mu = 0; mu < 4; mu++)
+        if (!by_mu[mu]) return fail(c, LYM_ERR_ARG, "null link array");
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    for (int mu = 0; mu < 4; mu++) {
+        LYM_HIP(c, hipMemcpyAsync(c->staging, by_mu[mu], sizeof(double) * 18 * V, hipMemcpyHostToDevice, c->stream));
+        if (c->prec == LYM_PREC_FP64)
+            lym::k_aos_to_soa<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+                c->staging, static_cast<double2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
+        else
+            lym::k_aos_to_soa<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+                c->staging, static_cast<float2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
+        LYM_LAUNCHED(c, "k_aos_to_soa");
+    }
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_download_links(lym_ctx *c, double *const by_mu[4])
+{
+    if (!c || !by_mu) return LYM_ERR_ARG;
+    for (int mu = 0; mu < 4; mu++)
+        if (!by_mu[mu]) return fail(c, LYM_ERR_ARG, "null link array");
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    for (int mu = 0; mu < 4; mu++) {
+        if (c->prec == LYM_PREC_FP64)
+            lym::k_soa_to_aos<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+                static_cast<const double2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, c->staging, V);
+        else
+            lym::k_soa_to_aos<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+                static_cast<const float2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, c->staging, V);
+        LYM_LAUNCHED(c, "k_soa_to_aos");
+        LYM_HIP(c, hipMemcpyAsync(by_mu[mu], c->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, c->stream));
+        LYM_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return LYM_OK;
+}
+
+int lym_set_cold(lym_ctx *c)
+{
+    if (!c) return LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    if (c->prec == LYM_PREC_FP64)
+        lym::k_cold<double><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<double2 *>(c->W[c->cur]), V);
+    else
+        lym::k_cold<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V);
+    LYM_LAUNCHED(c, "k_cold");
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_set_hot(lym_ctx *c, uint64_t seed)
+{
+    if (!c) return LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    if (c->prec == LYM_PREC_FP64)
+        lym::k_hot<double><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<double2 *>(c->W[c->cur]), V, seed);
+    else
+        lym::k_hot<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V, seed);
+    LYM_LAUNCHED(c, "k_hot");
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+static int export_force(lym_ctx *c, int mu, int what, double *out)
+{
+    if (!c || !out || mu < 0 || mu > 3) return c ? fail(c, LYM_ERR_ARG, "bad mu or null output") : LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    if (c->prec == LYM_PREC_FP64)
+        lym::k_export_force<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+            static_cast<const double2 *>(c->W[c->cur]), c->g, mu, what, c->staging);
+    else
+        lym::k_export_force<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+            static_cast<const float2 *>(c->W[c->cur]), c->g, mu, what, c->staging);
+    LYM_LAUNCHED(c, "k_export_force");
+    LYM_HIP(c, hipMemcpyAsync(out, c->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, c->stream));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_staples(lym_ctx *c, int mu, double *omega_out) { return export_force(c, mu, 0, omega_out); }
+int lym_derivative(lym_ctx *c, int mu, double *z_out) { return export_force(c, mu, 1, z_out); }
+
+int lym_observables(lym_ctx *c, double out[4])
+{
+    if (!c || !out) return LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    int rc = observables(c, 0);
+    if (rc) return rc;
+    double raw[4];
+    LYM_HIP(c, hipMemcpyAsync(raw, c->obs, sizeof raw, hipMemcpyDeviceToHost, c->stream));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    normalise(c, raw, out);
+    return LYM_OK;
+}
+
+int lym_flow_steps(lym_ctx *c, double eps, int nsteps)
+{
+    if (!c || nsteps < 0 || !(eps > 0)) return c ? fail(c, LYM_ERR_ARG, "bad eps or nsteps") : LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    for (int i = 0; i < nsteps; i++) {
+        int rc = step(c, eps);
+        if (rc) return rc;
+    }
+    return LYM_OK;
+}
+
+int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *series, int max_rows, int *rows_out)
+{
+    if (!c || !series || measure_every < 1 || !(eps > 0) || !(tau > 0))
+        return c ? fail(c, LYM_ERR_ARG, "bad flow arguments") : LYM_ERR_ARG;
+    const int rows = int(tau / eps);  // src/Apps/wilsonflow.cpp:117
+    if (rows_out) *rows_out = rows;
+    if (rows < 1 || rows > max_rows) return fail(c, LYM_ERR_ARG, "series too small for int(tau/eps) rows");
+    LYM_HIP(c, hipSetDevice(c->device));
+    const int nmeas = 1 + (rows - 1) / measure_every;
+    int rc = ensure_slots(c, nmeas);
+    if (rc) return rc;
+    int slot = 0;
+    if ((rc = observables(c, slot++))) return rc;
+    for (int t = 1; t < rows; t++) {
+        if ((rc = step(c, eps))) return rc;
+        if (t % measure_every == 0)
+            if ((rc = observables(c, slot++))) return rc;
+    }
+    std::vector<double> raw(4 * size_t(nmeas));
+    LYM_HIP(c, hipMemcpyAsync(raw.data(), c->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, c->stream));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    double tcur = 0.0;
+    slot = 0;
+    for (int t = 0; t < rows; t++) {
+        if (t > 0) tcur += eps;  // src/Apps/wilsonflow.cpp:126
+        double *row = series + 4 * size_t(t);
+        row[0] = tcur;
+        if (t % measure_every == 0) {
+            double o[4];
+            normalise(c, &raw[4 * size_t(slot++)], o);
+            row[1] = o[0];
+            row[2] = o[2];
+            row[3] = o[1];
+        } else {
+            row[1] = row[2] = row[3] = NAN;
+        }
+    }
+    return LYM_OK;
+}
+
+int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,
+                   uint64_t seed, const double *predrawn, int64_t *accepted)
+{
+    if (!c) return LYM_ERR_ARG;
+    if (nhit < 1 || nsweeps < 0 || first_sweep < 0 || (candidate != LYM_CAND_SU2 && candidate != LYM_CAND_EXPH) ||
+        !(eps > 0 && eps < 1))
+        return fail(c, LYM_ERR_ARG, "bad Metropolis arguments");
+    for (int d = 0; d < 4; d++)
+        if (c->dims[d] % 2) return fail(c, LYM_ERR_ARG, "checkerboard needs even extents");
+    LYM_HIP(c, hipSetDevice(c->device));
+    const long V = c->g.V;
+    const int nu = candidate == LYM_CAND_SU2 ? 13 : 9;
+    const size_t per_sweep = size_t(4) * V * nhit * nu;
+    if (predrawn) {
+        const size_t need = per_sweep * nsweeps;
+        if (need > c->predrawn_cap) {
+            if (c->predrawn) hipFree(c->predrawn);
+            c->predrawn = nullptr;
+            c->predrawn_cap = 0;
+            LYM_HIP(c, hipMalloc(&c->predrawn, sizeof(double) * need));
+            c->predrawn_cap = need;
+        }
+        LYM_HIP(c, hipMemcpyAsync(c->predrawn, predrawn, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+    }
+    lym::k_zero_u64<<<1, 64, 0, c->stream>>>(c->accepted, 1);
+    LYM_LAUNCHED(c, "k_zero_u64");
+    const unsigned nb = blocks_for(V / 2, 256);
+    for (int sw = 0; sw < nsweeps; sw++) {
+        lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, seed,
+                           predrawn ? c->predrawn + per_sweep * sw : nullptr};
+        for (int mu = 0; mu < 4; mu++)
+            for (int par = 0; par < 2; par++) {
+                p.parity = par;
+                if (c->prec == LYM_PREC_FP64)
+                    lym::k_metropolis<double><<<nb, 256, 0, c->stream>>>(static_cast<double2 *>(c->W[c->cur]), c->g,
+                                                                          mu, p, c->accepted);
+                else
+                    lym::k_metropolis<float><<<nb, 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), c->g, mu,
+                                                                         p, c->accepted);
+                LYM_LAUNCHED(c, "k_metropolis");
+            }
+    }
+    unsigned long long acc = 0;
+    LYM_HIP(c, hipMemcpyAsync(&acc, c->accepted, sizeof acc, hipMemcpyDeviceToHost, c->stream));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    if (accepted) *accepted = int64_t(acc);
+    return LYM_OK;
+}
+
+int lym_synchronize(lym_ctx *c)
+{
+    if (!c) return LYM_ERR_ARG;
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return LYM_OK;
+}
+
+int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, double *total_ms, double stage_ms[4])
+{
+    if (!c || nsteps < 1 || measure_every < 0 || !stage_ms) return c ? fail(c, LYM_ERR_ARG, "bad profile args") : LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    const int nmeas = measure_every ? nsteps / measure_every : 0;
+    const size_t nev = 2 + 4 * size_t(nsteps) + 2 * size_t(nmeas);
+    int rc = ensure_events(c, nev);
+    if (rc) return rc;
+    hipEvent_t *ev = c->events.data();
+    LYM_HIP(c, hipEventRecord(ev[0], c->stream));
+    size_t k = 2;
+    std::vector<size_t> obs_at;
+    for (int i = 1; i <= nsteps; i++) {
+        if ((rc = step(c, eps, ev + k))) return rc;
+        k += 4;
+        if (measure_every && i % measure_every == 0) {
+            obs_at.push_back(k);
+            if ((rc = observables(c, 0, ev + k))) return rc;
+            k += 2;
+        }
+    }
+    LYM_HIP(c, hipEventRecord(ev[1], c->stream));
+    LYM_HIP(c, hipEventSynchronize(ev[1]));
+    float ms = 0;
+    LYM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    if (total_ms) *total_ms = ms;
+    double st[3] = {0, 0, 0};
+    k = 2;
+    for (int i = 1; i <= nsteps; i++) {
+        for (int j = 0; j < 3; j++) {
+            LYM_HIP(c, hipEventElapsedTime(&ms, ev[k + j], ev[k + j + 1]));
+            st[j] += ms;
+        }
+        k += 4;
+        if (measure_every && i % measure_every == 0) k += 2;
+    }
+    double so = 0;
+    for (size_t at : obs_at) {
+        LYM_HIP(c, hipEventElapsedTime(&ms, ev[at], ev[at + 1]));
+        so += ms;
+    }
+    for (int j = 0; j < 3; j++) stage_ms[j] = st[j] / nsteps;
+    stage_ms[3] = obs_at.empty() ? 0.0 : so / double(obs_at.size());
+    return LYM_OK;
+}
+
+}  // extern "C"

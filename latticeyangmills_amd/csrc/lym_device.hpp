@@ -1,0 +1,506 @@
+// Device-side SU(3) algebra, su(3) exponential, lattice geometry, Philox and
+// compensated reductions for gfx950.
+//
+// Link storage in HBM is structure-of-arrays by complex element:
+//   field[(mu*9 + e)*V + s]   (e = 3*i + j, one complex per slot)
+// so a wavefront of 64 consecutive sites reads 64 consecutive complexes
+// (1 KiB FP64 / 512 B FP32) per element.  The reference keeps each link as
+// 18 interleaved doubles per site (include/Math/su3.h:63); conversion happens
+// once at upload/download.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lym {
+
+// ---------------------------------------------------------------- types ---
+
+template <typename R> struct cplx {
+    R re, im;
+};
+
+template <typename R> struct vec2;
+template <> struct vec2<double> { using type = double2; };
+template <> struct vec2<float> { using type = float2; };
+template <typename R> using vec2_t = typename vec2<R>::type;
+
+// 3x3 complex matrix, row-major, e[3*i + j]
+template <typename R> struct su3 {
+    cplx<R> e[9];
+};
+
+// Hermitian traceless 3x3 stored as its 8 real su(3) coordinates:
+// diagonal d0, d1 (d2 = -d0 - d1), upper off-diagonal o01, o02, o12.
+template <typename R> struct herm8 {
+    R d0, d1;
+    cplx<R> o01, o02, o12;
+};
+
+// Anti-Hermitian traceless 3x3: diagonal i*a0, i*a1, i*a2 (a0+a1+a2 = 0 up
+// to rounding), upper off-diagonal o01, o02, o12, lower = -conj(upper).
+template <typename R> struct ah8 {
+    R a0, a1, a2;
+    cplx<R> o01, o02, o12;
+};
+
+#define LYM_DI __device__ __forceinline__
+
+template <typename R> LYM_DI cplx<R> cmk(R a, R b) { return cplx<R>{a, b}; }
+template <typename R> LYM_DI cplx<R> conj(cplx<R> a) { return cplx<R>{a.re, -a.im}; }
+
+// acc + a*b
+template <typename R> LYM_DI cplx<R> cfma(cplx<R> a, cplx<R> b, cplx<R> acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+    return acc;
+}
+// acc + a*conj(b)
+template <typename R> LYM_DI cplx<R> cfma_nc(cplx<R> a, cplx<R> b, cplx<R> acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.im, b.re, acc.im);
+    acc.im = fma(-a.re, b.im, acc.im);
+    return acc;
+}
+// acc + conj(a)*b
+template <typename R> LYM_DI cplx<R> cfma_cn(cplx<R> a, cplx<R> b, cplx<R> acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+    return acc;
+}
+// acc + conj(a)*conj(b)
+template <typename R> LYM_DI cplx<R> cfma_cc(cplx<R> a, cplx<R> b, cplx<R> acc)
+{
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(-a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+    return acc;
+}
+
+template <typename R> LYM_DI su3<R> zero3()
+{
+    su3<R> r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.e[k] = cplx<R>{R(0), R(0)};
+    return r;
+}
+
+// r = a * b
+template <typename R> LYM_DI su3<R> mul_nn(const su3<R> &a, const su3<R> &b)
+{
+    su3<R> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s{a.e[3 * i].re * b.e[j].re, a.e[3 * i].re * b.e[j].im};
+            s.re = fma(-a.e[3 * i].im, b.e[j].im, s.re);
+            s.im = fma(a.e[3 * i].im, b.e[j].re, s.im);
+            s = cfma(a.e[3 * i + 1], b.e[3 + j], s);
+            s = cfma(a.e[3 * i + 2], b.e[6 + j], s);
+            r.e[3 * i + j] = s;
+        }
+    return r;
+}
+// r = a * b^+
+template <typename R> LYM_DI su3<R> mul_na(const su3<R> &a, const su3<R> &b)
+{
+    su3<R> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s{R(0), R(0)};
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = cfma_nc(a.e[3 * i + k], b.e[3 * j + k], s);
+            r.e[3 * i + j] = s;
+        }
+    return r;
+}
+// r = a^+ * b
+template <typename R> LYM_DI su3<R> mul_an(const su3<R> &a, const su3<R> &b)
+{
+    su3<R> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s{R(0), R(0)};
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = cfma_cn(a.e[3 * k + i], b.e[3 * k + j], s);
+            r.e[3 * i + j] = s;
+        }
+    return r;
+}
+// acc += a * b^+
+template <typename R> LYM_DI void acc_na(su3<R> &acc, const su3<R> &a, const su3<R> &b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s = acc.e[3 * i + j];
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = cfma_nc(a.e[3 * i + k], b.e[3 * j + k], s);
+            acc.e[3 * i + j] = s;
+        }
+}
+// acc += a^+ * b
+template <typename R> LYM_DI void acc_an(su3<R> &acc, const su3<R> &a, const su3<R> &b)
+{
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s = acc.e[3 * i + j];
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = cfma_cn(a.e[3 * k + i], b.e[3 * k + j], s);
+            acc.e[3 * i + j] = s;
+        }
+}
+template <typename R> LYM_DI su3<R> adj(const su3<R> &a)
+{
+    su3<R> r;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) r.e[3 * i + j] = conj(a.e[3 * j + i]);
+    return r;
+}
+template <typename R> LYM_DI R retrace(const su3<R> &a) { return a.e[0].re + a.e[4].re + a.e[8].re; }
+
+// Re Tr(a * b)
+template <typename R> LYM_DI R retr_mul(const su3<R> &a, const su3<R> &b)
+{
+    R s = R(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s = fma(a.e[3 * i + k].re, b.e[3 * k + i].re, s);
+            s = fma(-a.e[3 * i + k].im, b.e[3 * k + i].im, s);
+        }
+    return s;
+}
+
+template <typename To, typename From> LYM_DI su3<To> convert(const su3<From> &a)
+{
+    su3<To> r;
+#pragma unroll
+    for (int k = 0; k < 9; k++) r.e[k] = cplx<To>{To(a.e[k].re), To(a.e[k].im)};
+    return r;
+}
+
+// ------------------------------------------------------------ HBM access ---
+
+template <typename R, typename S>
+LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, long s)
+{
+    su3<R> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        vec2_t<S> v = plane[k * V + s];
+        m.e[k] = cplx<R>{R(v.x), R(v.y)};
+    }
+    return m;
+}
+
+template <typename S, typename R>
+LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, long s, const su3<R> &m)
+{
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        vec2_t<S> v;
+        v.x = S(m.e[k].re);
+        v.y = S(m.e[k].im);
+        plane[k * V + s] = v;
+    }
+}
+
+// ------------------------------------------------------------- geometry ---
+
+// Lexicographic site index s = x + Lx*(y + Ly*(z + Lz*t)), periodic
+// boundaries in every direction (one-rank semantics of the reference's
+// halo shift, include/Math/lattice.h:360-594).
+struct Geom {
+    int L[4];
+    int stride[4];
+    long V;
+};
+
+// Neighbour offsets of one site: fwd[d] = index(x + e_d) - s, bwd[d] = index(x - e_d) - s.
+struct Nbr {
+    int fwd[4], bwd[4];
+};
+
+LYM_DI Nbr neighbours(const Geom g, long s)
+{
+    Nbr n;
+    long r = s;
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        int c = (d < 3) ? int(r % g.L[d]) : int(r);
+        if (d < 3) r /= g.L[d];
+        n.fwd[d] = (c == g.L[d] - 1) ? -(g.L[d] - 1) * g.stride[d] : g.stride[d];
+        n.bwd[d] = (c == 0) ? (g.L[d] - 1) * g.stride[d] : -g.stride[d];
+    }
+    return n;
+}
+
+// ------------------------------------------------- su(3) exponential map ---
+
+// Q^2 for Hermitian traceless Q (Hermitian result: diagonal real parts in
+// q2.e[0,4,8].re, upper triangle in e[1], e[2], e[5]; lower filled as conj).
+LYM_DI su3<double> herm_square(const herm8<double> &q)
+{
+    double d2 = -q.d0 - q.d1;
+    double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
+    double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
+    double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
+    su3<double> r;
+    r.e[0] = cplx<double>{q.d0 * q.d0 + n01 + n02, 0.0};
+    r.e[4] = cplx<double>{q.d1 * q.d1 + n01 + n12, 0.0};
+    r.e[8] = cplx<double>{d2 * d2 + n02 + n12, 0.0};
+    // (Q^2)_01 = (d0 + d1) o01 + o02 conj(o12)
+    cplx<double> t01{(q.d0 + q.d1) * q.o01.re, (q.d0 + q.d1) * q.o01.im};
+    t01 = cfma_nc(q.o02, q.o12, t01);
+    // (Q^2)_02 = (d0 + d2) o02 + o01 o12
+    cplx<double> t02{(q.d0 + d2) * q.o02.re, (q.d0 + d2) * q.o02.im};
+    t02 = cfma(q.o01, q.o12, t02);
+    // (Q^2)_12 = (d1 + d2) o12 + conj(o01) o02
+    cplx<double> t12{(q.d1 + d2) * q.o12.re, (q.d1 + d2) * q.o12.im};
+    t12 = cfma_cn(q.o01, q.o02, t12);
+    r.e[1] = t01;
+    r.e[2] = t02;
+    r.e[5] = t12;
+    r.e[3] = conj(t01);
+    r.e[6] = conj(t02);
+    r.e[7] = conj(t12);
+    return r;
+}
+
+// exp(iQ) for Hermitian traceless Q: Cayley-Hamilton form of Morningstar &
+// Peardon (hep-lat/0311018), the algorithm of the reference's exp(SU3)
+// (src/Math/su3.cpp:96-183): c0 = det Q = Tr Q^3 / 3, c1 = Tr Q^2 / 2,
+// theta = acos(c0/c0max), u, w, xi0(w) (series for |w| < 0.05), h0..h2,
+// the c0 < 0 symmetry, f_j = h_j / (9u^2 - w^2), exp(iQ) = f0 + f1 Q + f2 Q^2.
+// Q == 0 gives the identity (the reference divides 0/0 there), and c0/c0max
+// is clamped to 1 against rounding.
+LYM_DI su3<double> exp_iq(const herm8<double> &q)
+{
+    const double d2 = -q.d0 - q.d1;
+    su3<double> q2 = herm_square(q);
+    const double c1raw = 0.5 * (q2.e[0].re + q2.e[4].re + q2.e[8].re);
+    const bool tiny = !(c1raw > 1e-40);  // |Q| < 1e-20: exp(iQ) = 1 + iQ to double precision
+    const double c1 = tiny ? 1.0 : c1raw;
+    // det Q = d0 d1 d2 + 2 Re(o01 o12 conj(o02)) - d0|o12|^2 - d1|o02|^2 - d2|o01|^2
+    cplx<double> p = cfma(q.o01, q.o12, cplx<double>{0.0, 0.0});
+    double re3 = p.re * q.o02.re + p.im * q.o02.im;
+    double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
+    double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
+    double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
+    double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
+    bool c0neg = c0 < 0.0;
+    c0 = fabs(c0);
+    const double inv_sqrt3 = 0.57735026918962576451;
+    double c1sqrt = sqrt(c1);
+    double c0max = 2.0 * c1 * c1sqrt * inv_sqrt3 / 3.0;
+    double ratio = fmin(c0 / c0max, 1.0);
+    double theta = acos(ratio);
+    double st, ct;
+    sincos(theta * (1.0 / 3.0), &st, &ct);
+    double w = c1sqrt * st;
+    double u = c1sqrt * inv_sqrt3 * ct;
+    double u2 = u * u, w2 = w * w;
+    double sw, cw;
+    sincos(w, &sw, &cw);
+    double xi = (fabs(w) < 0.05) ? 1.0 - w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0)) : sw / w;
+    double su, cu;
+    sincos(u, &su, &cu);
+    double cmu = cu, smu = -su;  // e^{-iu}
+    double c2u = cmu * cmu - smu * smu;
+    double s2u = -2.0 * cmu * smu;  // e^{2iu}
+    double cwcmu = cw * cmu, cwsmu = cw * smu, xicmu = xi * cmu, xismu = xi * smu;
+    double h0r = (u2 - w2) * c2u + 8.0 * u2 * cwcmu - 2.0 * u * (3.0 * u2 + w2) * xismu;
+    double h0i = (u2 - w2) * s2u + 8.0 * u2 * cwsmu + 2.0 * u * (3.0 * u2 + w2) * xicmu;
+    double h1r = 2.0 * u * c2u - 2.0 * u * cwcmu - (3.0 * u2 - w2) * xismu;
+    double h1i = 2.0 * u * s2u - 2.0 * u * cwsmu + (3.0 * u2 - w2) * xicmu;
+    double h2r = c2u - cwcmu + 3.0 * u * xismu;
+    double h2i = s2u - cwsmu - 3.0 * u * xicmu;
+    if (c0neg) {  // f_j(-c0) = (-1)^j conj f_j(c0)
+        h0i = -h0i;
+        h1r = -h1r;
+        h2i = -h2i;
+    }
+    double nrm = 1.0 / (9.0 * u2 - w2);
+    cplx<double> f0{h0r * nrm, h0i * nrm}, f1{h1r * nrm, h1i * nrm}, f2{h2r * nrm, h2i * nrm};
+    if (tiny) {
+        f0 = cplx<double>{1.0, 0.0};
+        f1 = cplx<double>{0.0, 1.0};
+        f2 = cplx<double>{0.0, 0.0};
+    }
+    // E = f0 + f1 Q + f2 Q^2 with Q, Q^2 Hermitian
+    su3<double> r;
+    r.e[0] = cplx<double>{f0.re + f1.re * q.d0 + f2.re * q2.e[0].re, f0.im + f1.im * q.d0 + f2.im * q2.e[0].re};
+    r.e[4] = cplx<double>{f0.re + f1.re * q.d1 + f2.re * q2.e[4].re, f0.im + f1.im * q.d1 + f2.im * q2.e[4].re};
+    r.e[8] = cplx<double>{f0.re + f1.re * d2 + f2.re * q2.e[8].re, f0.im + f1.im * d2 + f2.im * q2.e[8].re};
+    r.e[1] = cfma(f2, q2.e[1], cfma(f1, q.o01, cplx<double>{0.0, 0.0}));
+    r.e[2] = cfma(f2, q2.e[2], cfma(f1, q.o02, cplx<double>{0.0, 0.0}));
+    r.e[5] = cfma(f2, q2.e[5], cfma(f1, q.o12, cplx<double>{0.0, 0.0}));
+    r.e[3] = cfma(f2, q2.e[3], cfma(f1, conj(q.o01), cplx<double>{0.0, 0.0}));
+    r.e[6] = cfma(f2, q2.e[6], cfma(f1, conj(q.o02), cplx<double>{0.0, 0.0}));
+    r.e[7] = cfma(f2, q2.e[7], cfma(f1, conj(q.o12), cplx<double>{0.0, 0.0}));
+    return r;
+}
+
+// Z = (i/2) [X - X^+]_traceless for X = U * Omega
+// (src/Actions/puregauge.cpp:62-79, there formed as M = Omega^+ U^+, D = M^+ - M).
+template <typename R> LYM_DI herm8<double> force_from_x(const su3<R> &X)
+{
+    double i0 = double(X.e[0].im), i1 = double(X.e[4].im), i2 = double(X.e[8].im);
+    double tr = (i0 + i1 + i2) * (1.0 / 3.0);
+    herm8<double> z;
+    z.d0 = -(i0 - tr);
+    z.d1 = -(i1 - tr);
+    // Z_ij = (i/2)(X_ij - conj(X_ji)) for i < j
+    auto off = [](cplx<R> a, cplx<R> b) {
+        return cplx<double>{-0.5 * (double(a.im) + double(b.im)), 0.5 * (double(a.re) - double(b.re))};
+    };
+    z.o01 = off(X.e[1], X.e[3]);
+    z.o02 = off(X.e[2], X.e[6]);
+    z.o12 = off(X.e[5], X.e[7]);
+    return z;
+}
+
+LYM_DI herm8<double> h8_axpby(double a, const herm8<double> &x, double b, const herm8<double> &y)
+{
+    herm8<double> r;
+    r.d0 = a * x.d0 + b * y.d0;
+    r.d1 = a * x.d1 + b * y.d1;
+    r.o01 = cplx<double>{a * x.o01.re + b * y.o01.re, a * x.o01.im + b * y.o01.im};
+    r.o02 = cplx<double>{a * x.o02.re + b * y.o02.re, a * x.o02.im + b * y.o02.im};
+    r.o12 = cplx<double>{a * x.o12.re + b * y.o12.re, a * x.o12.im + b * y.o12.im};
+    return r;
+}
+LYM_DI herm8<double> h8_scale(double a, const herm8<double> &x)
+{
+    herm8<double> r;
+    r.d0 = a * x.d0;
+    r.d1 = a * x.d1;
+    r.o01 = cplx<double>{a * x.o01.re, a * x.o01.im};
+    r.o02 = cplx<double>{a * x.o02.re, a * x.o02.im};
+    r.o12 = cplx<double>{a * x.o12.re, a * x.o12.im};
+    return r;
+}
+
+// RK accumulator in HBM: acc[(mu*8 + k)*V + s], k = d0, d1, o01.re, o01.im, o02.re, o02.im, o12.re, o12.im
+template <typename S> LYM_DI herm8<double> load_h8(const S *__restrict__ a, long V, long s)
+{
+    herm8<double> z;
+    z.d0 = double(a[0 * V + s]);
+    z.d1 = double(a[1 * V + s]);
+    z.o01 = cplx<double>{double(a[2 * V + s]), double(a[3 * V + s])};
+    z.o02 = cplx<double>{double(a[4 * V + s]), double(a[5 * V + s])};
+    z.o12 = cplx<double>{double(a[6 * V + s]), double(a[7 * V + s])};
+    return z;
+}
+template <typename S> LYM_DI void store_h8(S *__restrict__ a, long V, long s, const herm8<double> &z)
+{
+    a[0 * V + s] = S(z.d0);
+    a[1 * V + s] = S(z.d1);
+    a[2 * V + s] = S(z.o01.re);
+    a[3 * V + s] = S(z.o01.im);
+    a[4 * V + s] = S(z.o02.re);
+    a[5 * V + s] = S(z.o02.im);
+    a[6 * V + s] = S(z.o12.re);
+    a[7 * V + s] = S(z.o12.im);
+}
+
+// Project a near-unitary matrix back onto SU(3): Gram-Schmidt on rows 0, 1,
+// row 2 = conj(row0 x row1) (so det = 1).
+LYM_DI void reunitarize(su3<double> &m)
+{
+    double n0 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) n0 += m.e[j].re * m.e[j].re + m.e[j].im * m.e[j].im;
+    double r0 = rsqrt(n0);
+    r0 = r0 * (1.5 - 0.5 * n0 * r0 * r0);  // one Newton step for full FP64 accuracy
+#pragma unroll
+    for (int j = 0; j < 3; j++) m.e[j] = cplx<double>{m.e[j].re * r0, m.e[j].im * r0};
+    cplx<double> p{0.0, 0.0};  // <row0|row1>
+#pragma unroll
+    for (int j = 0; j < 3; j++) p = cfma_cn(m.e[j], m.e[3 + j], p);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        cplx<double> t = cfma(p, m.e[j], cplx<double>{0.0, 0.0});
+        m.e[3 + j] = cplx<double>{m.e[3 + j].re - t.re, m.e[3 + j].im - t.im};
+    }
+    double n1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) n1 += m.e[3 + j].re * m.e[3 + j].re + m.e[3 + j].im * m.e[3 + j].im;
+    double r1 = rsqrt(n1);
+    r1 = r1 * (1.5 - 0.5 * n1 * r1 * r1);
+#pragma unroll
+    for (int j = 0; j < 3; j++) m.e[3 + j] = cplx<double>{m.e[3 + j].re * r1, m.e[3 + j].im * r1};
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        cplx<double> c = cfma(m.e[j1], m.e[3 + j2], cplx<double>{0.0, 0.0});
+        cplx<double> a = m.e[j2], b = m.e[3 + j1];
+        c.re -= a.re * b.re - a.im * b.im;
+        c.im -= a.re * b.im + a.im * b.re;
+        m.e[6 + j] = conj(c);
+    }
+}
+
+// ----------------------------------------------------------------- Philox ---
+
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+LYM_DI uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+LYM_DI double u32_unit(uint32_t r) { return double(r) * (1.0 / 4294967296.0); }
+
+// ------------------------------------------------- compensated reductions ---
+
+// (s, c) with value s + c; combined by Knuth's TwoSum so no rounding error of
+// the partial sums is lost (Kahan-Babuska compensation in tree form).
+struct ksum {
+    double s, c;
+};
+LYM_DI ksum kadd(ksum a, ksum b)
+{
+    double s = a.s + b.s;
+    double bp = s - a.s;
+    double err = (a.s - (s - bp)) + (b.s - bp);
+    return ksum{s, a.c + b.c + err};
+}
+LYM_DI ksum kshfl_xor(ksum a, int m)
+{
+    return ksum{__shfl_xor(a.s, m, 64), __shfl_xor(a.c, m, 64)};
+}
+LYM_DI ksum wave_ksum(ksum a)
+{
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) a = kadd(a, kshfl_xor(a, m));
+    return a;
+}
+
+}  // namespace lym

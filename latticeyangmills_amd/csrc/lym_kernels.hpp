@@ -1,0 +1,551 @@
+// Per-site device routines and the kernels of the single-lattice path.
+//
+// Thread mapping of the flow stage: a 256-thread workgroup covers 64
+// consecutive sites, wave w = direction mu = w.  The four waves of a
+// workgroup therefore read overlapping neighbour links and share them in the
+// CU's L1 / the XCD's L2; each wave's loads of one matrix element are 64
+// consecutive complexes.
+#pragma once
+#include "lym_device.hpp"
+
+namespace lym {
+
+constexpr int kSitesPerBlock = 64;
+
+template <typename S> using F2 = vec2_t<S>;
+
+// ---------------------------------------------------------------- staple ---
+
+// Omega_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+//                              + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu) ]
+// (src/Actions/puregauge.cpp:36-54), so that U_mu(x) Omega_mu(x) is the sum
+// of the six plaquettes through the link.
+template <typename R, typename S, int MU>
+LYM_DI su3<R> staple(const F2<S> *__restrict__ f, long V, long s, const Nbr &n)
+{
+    const F2<S> *Fmu = f + (long)MU * 9 * V;
+    su3<R> om = zero3<R>();
+#pragma unroll
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == MU) continue;
+        const F2<S> *Fnu = f + (long)nu * 9 * V;
+        {
+            su3<R> a = load_link<R, S>(Fnu, V, s + n.fwd[MU]);
+            su3<R> b = load_link<R, S>(Fmu, V, s + n.fwd[nu]);
+            su3<R> t = mul_na(a, b);
+            su3<R> c = load_link<R, S>(Fnu, V, s);
+            acc_na(om, t, c);
+        }
+        {
+            const long y = s + n.bwd[nu];  // same mu-coordinate as s
+            su3<R> a = load_link<R, S>(Fmu, V, y);
+            su3<R> b = load_link<R, S>(Fnu, V, y + n.fwd[MU]);
+            su3<R> t = mul_nn(a, b);
+            su3<R> c = load_link<R, S>(Fnu, V, y);
+            acc_an(om, t, c);
+        }
+    }
+    return om;
+}
+
+// ---------------------------------------------------- flow: one RK stage ---
+
+// One Runge-Kutta-Munthe-Kaas stage for one link (src/Apps/wilsonflow.cpp:141-172):
+//   STAGE 1: acc = Z(W0)                     ; W1 = exp(i eps/4 acc) W0
+//   STAGE 2: acc = 8/9 Z(W1) - 17/36 acc     ; W2 = exp(i eps acc) W1
+//   STAGE 3: q   = 3/4 Z(W2) - acc           ; W3 = exp(i eps q) W2
+// Links are read from win and written to wout (all four directions of a
+// stage see the same input field, as the reference computes all four Z
+// before any update).  Staple and force run in the storage precision S;
+// the RK combination, the exponential and exp*U run in FP64.  REUNIT
+// projects the result back onto SU(3) before it is stored (mixed path).
+template <typename S, int STAGE, bool REUNIT, int MU>
+LYM_DI void flow_link(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
+                      const Geom g, long s, double eps)
+{
+    const long V = g.V;
+    Nbr n = neighbours(g, s);
+    su3<S> om = staple<S, S, MU>(win, V, s, n);
+    su3<S> U = load_link<S, S>(win + (long)MU * 9 * V, V, s);
+    su3<S> X = mul_nn(U, om);
+    herm8<double> z = force_from_x(X);
+    S *am = acc + (long)MU * 8 * V;
+    herm8<double> q;
+    if constexpr (STAGE == 1) {
+        store_h8<S>(am, V, s, z);
+        q = h8_scale(eps * 0.25, z);
+    } else if constexpr (STAGE == 2) {
+        herm8<double> a = load_h8<S>(am, V, s);
+        a = h8_axpby(8.0 / 9.0, z, -17.0 / 36.0, a);
+        store_h8<S>(am, V, s, a);
+        q = h8_scale(eps, a);
+    } else {
+        herm8<double> a = load_h8<S>(am, V, s);
+        q = h8_scale(eps, h8_axpby(0.75, z, -1.0, a));
+    }
+    su3<double> E = exp_iq(q);
+    su3<double> W = mul_nn(E, convert<double>(U));
+    if constexpr (REUNIT) reunitarize(W);
+    store_link<S, double>(wout + (long)MU * 9 * V, V, s, W);
+}
+
+template <typename S, int STAGE, bool REUNIT>
+LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
+                       const Geom g, long site0, double eps)
+{
+    const int mu = threadIdx.x >> 6;
+    const long s = site0 + (threadIdx.x & 63);
+    if (s >= g.V) return;
+    switch (mu) {
+    case 0: flow_link<S, STAGE, REUNIT, 0>(win, wout, acc, g, s, eps); break;
+    case 1: flow_link<S, STAGE, REUNIT, 1>(win, wout, acc, g, s, eps); break;
+    case 2: flow_link<S, STAGE, REUNIT, 2>(win, wout, acc, g, s, eps); break;
+    default: flow_link<S, STAGE, REUNIT, 3>(win, wout, acc, g, s, eps); break;
+    }
+}
+
+template <typename S, int STAGE, bool REUNIT>
+__global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
+                                                    S *__restrict__ acc, Geom g, double eps)
+{
+    flow_block<S, STAGE, REUNIT>(win, wout, acc, g, (long)blockIdx.x * kSitesPerBlock, eps);
+}
+
+// ----------------------------------------------------------- observables ---
+
+// Clover of plane (MU, NU) at site s (src/Observables/superobs.cpp:40-106):
+// clover = P_{mu nu}(x) + [(-mu,-nu) leaf] - [(+mu,-nu) leaf] - [(-mu,+nu) leaf]
+// with each leaf formed as in the reference; returns C = [clover - clover^+]
+// made traceless (8 reals), and Re Tr P_{mu nu}(x) in *plaq.
+template <typename R, typename S, int MU, int NU>
+LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, R *plaq)
+{
+    const F2<S> *Fm = f + (long)MU * 9 * V;
+    const F2<S> *Fn = f + (long)NU * 9 * V;
+    su3<R> cl;
+    {  // P = ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+
+        su3<R> t = mul_nn(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
+        t = mul_na(t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
+        cl = mul_na(t, load_link<R, S>(Fn, V, s));
+        *plaq = retrace(cl);
+    }
+    {  // y = x-mu-nu: ((U_mu(y+nu)^+ U_nu(y)^+) U_mu(y)) U_nu(y+mu) = (U_nu(y) U_mu(x-mu))^+ U_mu(y) U_nu(x-nu)
+        const long y = s + n.bwd[MU] + n.bwd[NU];
+        su3<R> t = mul_nn(load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
+        t = mul_an(t, load_link<R, S>(Fm, V, y));
+        t = mul_nn(t, load_link<R, S>(Fn, V, s + n.bwd[NU]));
+#pragma unroll
+        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re + t.e[k].re, cl.e[k].im + t.e[k].im};
+    }
+    {  // y = x-nu: ((U_mu(x) U_nu(x+mu-nu)^+) U_mu(y)^+) U_nu(y)
+        const long y = s + n.bwd[NU];
+        su3<R> t = mul_na(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
+        t = mul_na(t, load_link<R, S>(Fm, V, y));
+        t = mul_nn(t, load_link<R, S>(Fn, V, y));
+#pragma unroll
+        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re - t.e[k].re, cl.e[k].im - t.e[k].im};
+    }
+    {  // y = x-mu: ((U_mu(y)^+ U_nu(y)) U_mu(x-mu+nu)) U_nu(x)^+
+        const long y = s + n.bwd[MU];
+        su3<R> t = mul_an(load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
+        t = mul_nn(t, load_link<R, S>(Fm, V, y + n.fwd[NU]));
+        t = mul_na(t, load_link<R, S>(Fn, V, s));
+#pragma unroll
+        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re - t.e[k].re, cl.e[k].im - t.e[k].im};
+    }
+    // C = clover - clover^+ (anti-Hermitian), made traceless
+    ah8<R> C;
+    const R i0 = R(2) * cl.e[0].im, i1 = R(2) * cl.e[4].im, i2 = R(2) * cl.e[8].im;
+    const R tr = (i0 + i1 + i2) * R(1.0 / 3.0);
+    C.a0 = i0 - tr;
+    C.a1 = i1 - tr;
+    C.a2 = i2 - tr;
+    C.o01 = cplx<R>{cl.e[1].re - cl.e[3].re, cl.e[1].im + cl.e[3].im};
+    C.o02 = cplx<R>{cl.e[2].re - cl.e[6].re, cl.e[2].im + cl.e[6].im};
+    C.o12 = cplx<R>{cl.e[5].re - cl.e[7].re, cl.e[5].im + cl.e[7].im};
+    return C;
+}
+
+// Re Tr(A B) for anti-Hermitian A, B given by diagonal imaginary parts a_i and
+// upper off-diagonals (A_ji = -conj(A_ij)):  -sum_i a_i b_i - 2 sum_{i<j} Re(A_ij conj(B_ij)).
+// Re Tr(A B^+) = -Re Tr(A B).
+template <typename R> LYM_DI R retr_ah(const ah8<R> &A, const ah8<R> &B)
+{
+    R d = A.a0 * B.a0 + A.a1 * B.a1 + A.a2 * B.a2;
+    R o = A.o01.re * B.o01.re + A.o01.im * B.o01.im + A.o02.re * B.o02.re + A.o02.im * B.o02.im +
+          A.o12.re * B.o12.re + A.o12.im * B.o12.im;
+    return -d - R(2) * o;
+}
+
+// Contributions of one plane pair at one site (src/Observables/superobs.cpp:40-124):
+// PAIR 0 = planes (01, 23), 1 = (02, 13), 2 = (03, 12).  out[0] = Re Tr P over
+// both planes, out[1] = Re Tr(C C) over both planes (the reference's raw
+// energy terms, superobs.cpp:109), out[2] = this pair's term of
+// q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124).
+template <typename R, typename S, int PAIR>
+LYM_DI void pair_observables(const F2<S> *__restrict__ f, const Geom g, long s, double out[3])
+{
+    constexpr int M1 = 0, N1 = PAIR + 1;
+    constexpr int M2 = (PAIR == 2) ? 1 : (PAIR == 0 ? 2 : 1), N2 = (PAIR == 0) ? 3 : (PAIR == 1 ? 3 : 2);
+    const long V = g.V;
+    Nbr n = neighbours(g, s);
+    R p1, p2;
+    ah8<R> A = clover<R, S, M1, N1>(f, V, s, n, &p1);
+    ah8<R> B = clover<R, S, M2, N2>(f, V, s, n, &p2);
+    out[0] = double(p1) + double(p2);
+    out[1] = double(retr_ah(A, A)) + double(retr_ah(B, B));
+    const double t = double(retr_ah(A, B));
+    out[2] = (PAIR == 1) ? t : -t;  // -Re Tr(C02 C13^+) = +Re Tr(C02 C13)
+}
+
+// Block reduction of 4 compensated sums; thread 0 returns the block total.
+template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
+{
+    __shared__ ksum sh[NT / 64][4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = wave_ksum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 4; k++) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            ksum t = sh[0][k];
+            for (int i = 1; i < NT / 64; i++) t = kadd(t, sh[i][k]);
+            out[k] = t;
+        }
+    }
+}
+
+// 192-thread workgroups: 64 consecutive sites x 3 plane pairs (wave = pair).
+// Per-block compensated partials of {sum Re Tr P, sum Re Tr C^2, sum q,
+// sum |q terms|}.
+constexpr int kObsSites = 64;
+
+template <typename S>
+__global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, ksum *__restrict__ partials)
+{
+    const long s = (long)blockIdx.x * kObsSites + (threadIdx.x & 63);
+    const int pair = threadIdx.x >> 6;
+    ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (s < g.V) {
+        double o[3];
+        if (pair == 0)
+            pair_observables<S, S, 0>(f, g, s, o);
+        else if (pair == 1)
+            pair_observables<S, S, 1>(f, g, s, o);
+        else
+            pair_observables<S, S, 2>(f, g, s, o);
+        v[0].s = o[0];
+        v[1].s = o[1];
+        v[2].s = o[2];
+        v[3].s = fabs(o[2]);
+    }
+    block_ksum4<192>(v, partials + 4 * blockIdx.x);
+}
+
+// Sum nblk block partials into dst[0..3] (one workgroup).
+__global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__restrict__ partials, int nblk,
+                                                         double *__restrict__ dst)
+{
+    ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (int i = threadIdx.x; i < nblk; i += 256)
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = kadd(v[k], partials[4 * i + k]);
+    __shared__ ksum tot[4];
+    block_ksum4<256>(v, tot);
+    __syncthreads();
+    if (threadIdx.x < 4) dst[threadIdx.x] = tot[threadIdx.x].s + tot[threadIdx.x].c;
+}
+
+// ------------------------------------------------ layout and start fields ---
+
+// clears counters with a kernel (a memset node misbehaves on graph replays)
+__global__ void k_zero_u64(unsigned long long *p, int n)
+{
+    if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
+}
+
+// reference layout (V x 18 doubles for one mu) -> SoA plane of direction mu
+template <typename S>
+__global__ void k_aos_to_soa(const double *__restrict__ aos, F2<S> *__restrict__ plane, long V)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        F2<S> v;
+        v.x = S(aos[s * 18 + 2 * k]);
+        v.y = S(aos[s * 18 + 2 * k + 1]);
+        plane[k * V + s] = v;
+    }
+}
+
+template <typename S>
+__global__ void k_soa_to_aos(const F2<S> *__restrict__ plane, double *__restrict__ aos, long V)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        F2<S> v = plane[k * V + s];
+        aos[s * 18 + 2 * k] = double(v.x);
+        aos[s * 18 + 2 * k + 1] = double(v.y);
+    }
+}
+
+// all links = 1 (the reference's setToIdentity, src/Math/lattice.cpp:33-41)
+template <typename S> __global__ void k_cold(F2<S> *__restrict__ f, long V)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= 4 * V) return;
+    const long mu = l / V, s = l % V;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        F2<S> v;
+        v.x = (k == 0 || k == 4 || k == 8) ? S(1) : S(0);
+        v.y = S(0);
+        f[(mu * 9 + k) * V + s] = v;
+    }
+}
+
+// Haar-random SU(3) per link: two complex Gaussian rows (Box-Muller on
+// Philox uniforms, counter {link, 0xFFFFFFFF, 0, blk}, key {seed_lo,
+// seed_hi ^ 0x48414152}), Gram-Schmidt, row 2 = conj(row0 x row1).
+LYM_DI su3<double> haar_su3(uint32_t link, uint64_t seed)
+{
+    uint2 key = make_uint2(uint32_t(seed), uint32_t(seed >> 32) ^ 0x48414152u);
+    uint32_t r[12];
+#pragma unroll
+    for (uint32_t b = 0; b < 3; b++) {
+        uint4 o = philox4x32_10(make_uint4(link, 0xFFFFFFFFu, 0u, b), key);
+        r[4 * b] = o.x;
+        r[4 * b + 1] = o.y;
+        r[4 * b + 2] = o.z;
+        r[4 * b + 3] = o.w;
+    }
+    double gs[12];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        double u1 = (double(r[2 * k]) + 1.0) * (1.0 / 4294967296.0);
+        double u2 = u32_unit(r[2 * k + 1]);
+        double rad = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(6.283185307179586477 * u2, &sn, &cs);
+        gs[2 * k] = rad * cs;
+        gs[2 * k + 1] = rad * sn;
+    }
+    su3<double> m;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        m.e[j] = cplx<double>{gs[2 * j], gs[2 * j + 1]};
+        m.e[3 + j] = cplx<double>{gs[6 + 2 * j], gs[6 + 2 * j + 1]};
+    }
+    reunitarize(m);
+    return m;
+}
+
+template <typename S> __global__ void k_hot(F2<S> *__restrict__ f, long V, uint64_t seed)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= 4 * V) return;
+    const long mu = l / V, s = l % V;
+    su3<double> m = haar_su3(uint32_t(l), seed);
+    store_link<S, double>(f + mu * 9 * V, V, s, m);
+}
+
+// Staple (what = 0) or force Z (what = 1) of direction mu in the reference's
+// per-site layout (V x 18 doubles): the results of PureGauge::computeStaples
+// and PureGauge::computeDerivative (src/Actions/puregauge.cpp:36-79).
+template <typename S, int MU>
+LYM_DI void export_force_site(const F2<S> *__restrict__ f, const Geom g, long s, int what, double *__restrict__ out)
+{
+    Nbr n = neighbours(g, s);
+    su3<double> om = staple<double, S, MU>(f, g.V, s, n);
+    double *o = out + s * 18;
+    if (what == 0) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            o[2 * k] = om.e[k].re;
+            o[2 * k + 1] = om.e[k].im;
+        }
+        return;
+    }
+    su3<double> U = load_link<double, S>(f + (long)MU * 9 * g.V, g.V, s);
+    herm8<double> z = force_from_x(mul_nn(U, om));
+    const double d2 = -z.d0 - z.d1;
+    o[0] = z.d0;  o[1] = 0.0;
+    o[8] = z.d1;  o[9] = 0.0;
+    o[16] = d2;   o[17] = 0.0;
+    o[2] = z.o01.re;  o[3] = z.o01.im;  o[6] = z.o01.re;   o[7] = -z.o01.im;
+    o[4] = z.o02.re;  o[5] = z.o02.im;  o[12] = z.o02.re;  o[13] = -z.o02.im;
+    o[10] = z.o12.re; o[11] = z.o12.im; o[14] = z.o12.re;  o[15] = -z.o12.im;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_export_force(const F2<S> *__restrict__ f, Geom g, int mu, int what,
+                                                      double *__restrict__ out)
+{
+    const long s = (long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= g.V) return;
+    switch (mu) {
+    case 0: export_force_site<S, 0>(f, g, s, what, out); break;
+    case 1: export_force_site<S, 1>(f, g, s, what, out); break;
+    case 2: export_force_site<S, 2>(f, g, s, what, out); break;
+    default: export_force_site<S, 3>(f, g, s, what, out); break;
+    }
+}
+
+// ------------------------------------------------ checkerboard Metropolis ---
+
+// SU(2) subgroup element of randSU3Transf (src/Math/random.cpp:81-102).
+LYM_DI void su2_near_identity(const double *u4, double eps, double x[4])
+{
+    double norm = 0.0;
+    x[0] = u4[0] - 0.5;
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+        x[i] = u4[i] - 0.5;
+        norm += x[i] * x[i];
+    }
+    norm = sqrt(norm);
+    double x0 = sqrt(1.0 - eps * eps);
+    x[0] = (x[0] < 0) ? -x0 : x0;
+#pragma unroll
+    for (int i = 1; i < 4; i++) x[i] = x[i] * eps / norm;
+}
+
+// X = R S T (src/Math/random.cpp:71-156), SU(2) blocks
+// [[x0 + i x3, x2 + i x1], [-x2 + i x1, x0 - i x3]] in (0,1), (0,2), (1,2).
+LYM_DI su3<double> cand_su2_product(const double *u, double eps)
+{
+    double x[4];
+    su3<double> R = zero3<double>(), Sm = zero3<double>(), T = zero3<double>();
+    R.e[0].re = R.e[4].re = R.e[8].re = 1.0;
+    Sm.e[0].re = Sm.e[4].re = Sm.e[8].re = 1.0;
+    T.e[0].re = T.e[4].re = T.e[8].re = 1.0;
+    su2_near_identity(u, eps, x);
+    R.e[0] = cplx<double>{x[0], x[3]};
+    R.e[4] = cplx<double>{x[0], -x[3]};
+    R.e[1] = cplx<double>{x[2], x[1]};
+    R.e[3] = cplx<double>{-x[2], x[1]};
+    su2_near_identity(u + 4, eps, x);
+    Sm.e[0] = cplx<double>{x[0], x[3]};
+    Sm.e[8] = cplx<double>{x[0], -x[3]};
+    Sm.e[2] = cplx<double>{x[2], x[1]};
+    Sm.e[6] = cplx<double>{-x[2], x[1]};
+    su2_near_identity(u + 8, eps, x);
+    T.e[4] = cplx<double>{x[0], x[3]};
+    T.e[8] = cplx<double>{x[0], -x[3]};
+    T.e[5] = cplx<double>{x[2], x[1]};
+    T.e[7] = cplx<double>{-x[2], x[1]};
+    return mul_nn(mul_nn(R, Sm), T);
+}
+
+// X = exp(i eps H), H = sum_a (2u_a - 1) lambda_a (Gell-Mann matrices).
+LYM_DI su3<double> cand_exph(const double *u, double eps)
+{
+    double h[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) h[a] = 2.0 * u[a] - 1.0;
+    const double r3 = 0.57735026918962576451;
+    herm8<double> q;
+    q.d0 = eps * (h[2] + h[7] * r3);
+    q.d1 = eps * (-h[2] + h[7] * r3);
+    q.o01 = cplx<double>{eps * h[0], -eps * h[1]};
+    q.o02 = cplx<double>{eps * h[3], -eps * h[4]};
+    q.o12 = cplx<double>{eps * h[5], -eps * h[6]};
+    return exp_iq(q);
+}
+
+struct MetroParams {
+    double beta, eps;
+    int nhit, candidate, nu;  // nu = uniforms per hit (13 or 9)
+    uint32_t sweep;           // global sweep index (Philox counter)
+    int parity;
+    uint64_t seed;
+    const double *predrawn;  // test mode: this sweep's block, or nullptr
+};
+
+// Metropolis update of link (mu, s): nhit hits of U' = X U with
+// dS = -(beta/3) Re Tr[(U' - U) Omega], accepted iff exp(-dS) > u
+// (src/Apps/gaugefieldfactory.cpp:205-220, src/Actions/puregauge.cpp:28).
+// Arithmetic in FP64; FP32 storage is reunitarised before the store.
+template <typename S, int MU>
+LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroParams &p)
+{
+    const long V = g.V;
+    Nbr n = neighbours(g, s);
+    su3<double> om = staple<double, S, MU>(f, V, s, n);
+    F2<S> *Fm = f + (long)MU * 9 * V;
+    su3<double> U = load_link<double, S>(Fm, V, s);
+    const uint32_t lk = uint32_t(MU * V + s);
+    const uint2 key = make_uint2(uint32_t(p.seed), uint32_t(p.seed >> 32));
+    int acc = 0;
+    for (int hit = 0; hit < p.nhit; hit++) {
+        double u[16];
+        if (p.predrawn) {
+            const double *src = p.predrawn + ((long)lk * p.nhit + hit) * p.nu;
+#pragma unroll
+            for (int k = 0; k < 13; k++)
+                if (k < p.nu) u[k] = src[k];
+        } else {
+#pragma unroll
+            for (uint32_t b = 0; b < 4; b++) {
+                uint4 o = philox4x32_10(make_uint4(lk, p.sweep, uint32_t(hit), b), key);
+                u[4 * b] = u32_unit(o.x);
+                u[4 * b + 1] = u32_unit(o.y);
+                u[4 * b + 2] = u32_unit(o.z);
+                u[4 * b + 3] = u32_unit(o.w);
+            }
+        }
+        su3<double> X = (p.candidate == 0) ? cand_su2_product(u, p.eps) : cand_exph(u, p.eps);
+        su3<double> Un = mul_nn(X, U);
+        su3<double> d;
+#pragma unroll
+        for (int k = 0; k < 9; k++) d.e[k] = cplx<double>{Un.e[k].re - U.e[k].re, Un.e[k].im - U.e[k].im};
+        double dS = -p.beta / 3.0 * retr_mul(d, om);
+        const double uacc = (p.candidate == 0) ? u[12] : u[8];
+        if (exp(-dS) > uacc) {
+            U = Un;
+            acc++;
+        }
+    }
+    if constexpr (sizeof(S) == 4) reunitarize(U);
+    store_link<S, double>(Fm, V, s, U);
+    return acc;
+}
+
+// site index of the i-th site of the given parity (requires even extents)
+LYM_DI long parity_site(const Geom g, long i, int parity)
+{
+    const long half = g.L[0] / 2;
+    const long r = i / half, xh = i % half;
+    const long y = r % g.L[1], z = (r / g.L[1]) % g.L[2], t = r / ((long)g.L[1] * g.L[2]);
+    const long x = 2 * xh + ((parity + y + z + t) & 1);
+    return x + (long)g.L[0] * r;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_metropolis(F2<S> *__restrict__ f, Geom g, int mu, MetroParams p,
+                                                    unsigned long long *__restrict__ accepted)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    int acc = 0;
+    if (i < g.V / 2) {
+        const long s = parity_site(g, i, p.parity);
+        switch (mu) {
+        case 0: acc = metro_link<S, 0>(f, g, s, p); break;
+        case 1: acc = metro_link<S, 1>(f, g, s, p); break;
+        case 2: acc = metro_link<S, 2>(f, g, s, p); break;
+        default: acc = metro_link<S, 3>(f, g, s, p); break;
+        }
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepted, (unsigned long long)acc);
+}
+
+}  // namespace lym

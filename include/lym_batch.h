@@ -1,0 +1,61 @@
+/*
+ * lym_batch.h -- C ABI of the batched ensemble path: many independent
+ * periodic lattices of uneven sizes, flowed by one persistent launch per RK
+ * stage with the flow loop captured in a HIP graph.
+ *
+ * The reference processes its input configurations one after another
+ * (WilsonFlow::flowConfigurations, src/Apps/wilsonflow.cpp:96-109, one
+ * readConf + applyWilsonFlow per file); these entry points replace that loop
+ * for a whole ensemble at once.  Layout and error conventions as in lym.h.
+ */
+#ifndef LYM_BATCH_H
+#define LYM_BATCH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lym_batch lym_batch;
+
+#define LYM_SCHED_LINEAR 0 /* workgroup k takes tiles k, k+G, ... */
+#define LYM_SCHED_XCD 1    /* tiles renumbered so each XCD walks a contiguous run */
+
+/* nconf lattices, dims = nconf x 4 extents; all start cold.  The lattices'
+ * list replaces WilsonFlow's m_inputConfList (src/Apps/wilsonflow.cpp:96-109). */
+int lym_batch_create(lym_batch **b, int nconf, const int *dims, int precision, int device);
+int lym_batch_destroy(lym_batch *b);
+const char *lym_batch_last_error(const lym_batch *b);
+int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *sched);
+
+/* Per-lattice transfer in the reference's GluonField layout
+ * (InputConf::readConf, src/InputOutput/inputconf.cpp:114-155). */
+int lym_batch_upload_links(lym_batch *b, int conf, const double *const by_mu[4]);
+int lym_batch_download_links(lym_batch *b, int conf, double *const by_mu[4]);
+/* Haar-random hot start per lattice, seeds[c] as lym_set_hot. */
+int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds);
+/* Checkerboard Metropolis on every lattice at once (GaugeFieldFactory::MCUpdate,
+ * src/Apps/gaugefieldfactory.cpp:192-220), Philox keyed by seeds[c];
+ * accepted[c] = accepted hits of lattice c (may be NULL). */
+int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep,
+                         int candidate, const uint64_t *seeds, int64_t *accepted);
+/* SuperObs::compute for every lattice (src/Observables/superobs.cpp:31-128):
+ * out[4c + k] as lym_observables. */
+int lym_batch_observables(lym_batch *b, double *out);
+/* WilsonFlow::applyWilsonFlow for every lattice (src/Apps/wilsonflow.cpp:115-136):
+ * series[(c*max_rows + t)*4 + k] as lym_flow.  use_graph != 0 records the whole
+ * flow loop (all steps and measurements) as one HIP graph and replays it. */
+int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
+                   int max_rows, int *rows_out);
+/* Launch shape of the stage kernel: grid = 0 means one workgroup per 64-site
+ * tile, otherwise a persistent grid of that many workgroups. */
+int lym_batch_set_schedule(lym_batch *b, int grid, int sched);
+/* Times the candidate shapes on the batch itself and keeps the fastest;
+ * cache_path (may be NULL) holds one line per batch shape and precision.
+ * summary receives a one-line description of the choice. */
+int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

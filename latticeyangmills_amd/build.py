@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["lym_capi.hip"]
+LIB_SOURCES = ["lym_capi.hip", "lym_batch.hip"]
 
 
 def _run(cmd, cwd=None):

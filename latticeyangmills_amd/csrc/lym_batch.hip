@@ -1,0 +1,684 @@
+// Batched ensemble path: many independent lattices of uneven sizes flowed by
+// one persistent launch per RK stage, the whole flow loop captured in a HIP
+// graph, the launch shape picked by an autotuner (declared in include/lym_batch.h).
+//
+// Work list: every lattice c is cut into tiles of 64 consecutive sites; tile
+// t of the whole batch belongs to the lattice whose first tile index
+// tile0[c] <= t < tile0[c+1].  A workgroup (4 waves = 4 directions) takes a
+// tile at a time.  Two schedules map workgroups to tiles:
+//   LINEAR: tile = blockIdx.x + k * gridDim.x
+//   XCD:    consecutive workgroup ids are dispatched round-robin over the 8
+//           XCDs, so tile = (xcd * per_xcd + local) keeps runs of neighbouring
+//           tiles (which share halo links) in one XCD's L2.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "lym.h"
+#include "lym_batch.h"
+#include "lym_kernels.hpp"
+
+namespace lym {
+
+struct ConfDesc {
+    Geom g;
+    long link_off;  // offset in complexes of the lattice's field in the batch buffers
+    long acc_off;   // offset in reals of its RK accumulator
+    long tile0;     // first tile of the lattice
+    long half0;     // first Metropolis work item of the lattice (V/2 sites per sub-sweep)
+    uint64_t seed;
+};
+
+constexpr int kMaxConf = 1024;
+enum { SCHED_LINEAR = 0, SCHED_XCD = 1 };
+
+LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (d[mid].tile0 <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+LYM_DI long tile_of(long k, int sched, long ntiles)
+{
+    if (sched == SCHED_LINEAR) return k;
+    // k-th tile handled by the grid, renumbered so that XCD x (= k % 8)
+    // walks the contiguous chunk [x*per, (x+1)*per)
+    const long per = (ntiles + 7) / 8;
+    const long x = k & 7, j = k >> 3;
+    return x * per + j;
+}
+
+template <typename S, int STAGE, bool REUNIT>
+__global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
+                                                          S *__restrict__ acc, const ConfDesc *__restrict__ desc,
+                                                          int nconf, long ntiles, int sched, double eps)
+{
+    // ceil(ntiles/8)*8 slots so that every XCD chunk is walked completely
+    const long slots = (sched == SCHED_XCD) ? ((ntiles + 7) / 8) * 8 : ntiles;
+    for (long k = blockIdx.x; k < slots; k += gridDim.x) {
+        const long tile = tile_of(k, sched, ntiles);
+        if (tile >= ntiles) continue;
+        const int c = find_conf(desc, nconf, tile);
+        const ConfDesc d = desc[c];
+        flow_block<S, STAGE, REUNIT>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
+                                     (tile - d.tile0) * kSitesPerBlock, eps);
+    }
+}
+
+// observables: one partial per tile (192 threads = 64 sites x 3 plane pairs)
+template <typename S>
+__global__ __launch_bounds__(192) void k_batch_observables(const F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
+                                                           int nconf, long ntiles, ksum *__restrict__ partials)
+{
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int c = find_conf(desc, nconf, tile);
+        const ConfDesc d = desc[c];
+        const long s = (tile - d.tile0) * kObsSites + (threadIdx.x & 63);
+        const int pair = threadIdx.x >> 6;
+        ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+        if (s < d.g.V) {
+            double o[3];
+            const F2<S> *fc = f + d.link_off;
+            if (pair == 0)
+                pair_observables<S, S, 0>(fc, d.g, s, o);
+            else if (pair == 1)
+                pair_observables<S, S, 1>(fc, d.g, s, o);
+            else
+                pair_observables<S, S, 2>(fc, d.g, s, o);
+            v[0].s = o[0];
+            v[1].s = o[1];
+            v[2].s = o[2];
+            v[3].s = fabs(o[2]);
+        }
+        block_ksum4<192>(v, partials + 4 * tile);
+        __syncthreads();
+    }
+}
+
+// per-lattice sum of its tiles' partials; one workgroup per lattice
+static __global__ __launch_bounds__(256) void k_batch_reduce(const ksum *__restrict__ partials, const ConfDesc *__restrict__ desc,
+                                                      int nconf, long ntiles, double *__restrict__ dst)
+{
+    const int c = blockIdx.x;
+    const long t0 = desc[c].tile0, t1 = (c + 1 < nconf) ? desc[c + 1].tile0 : ntiles;
+    ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    for (long i = t0 + threadIdx.x; i < t1; i += 256)
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = kadd(v[k], partials[4 * i + k]);
+    __shared__ ksum tot[4];
+    block_ksum4<256>(v, tot);
+    __syncthreads();
+    if (threadIdx.x < 4) dst[4 * c + threadIdx.x] = tot[threadIdx.x].s + tot[threadIdx.x].c;
+}
+
+// checkerboard Metropolis over the batch: work item i of sub-sweep (mu, parity)
+// is half-site i - half0[c] of lattice c
+template <typename S>
+__global__ __launch_bounds__(256) void k_batch_metropolis(F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
+                                                          int nconf, long nhalf, int mu, MetroParams p,
+                                                          unsigned long long *__restrict__ accepted)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    int acc = 0, c = -1;
+    if (i < nhalf) {
+        int lo = 0, hi = nconf - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (desc[mid].half0 <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        c = lo;
+        const ConfDesc d = desc[c];
+        MetroParams q = p;
+        q.seed = d.seed;
+        const long s = parity_site(d.g, i - d.half0, p.parity);
+        F2<S> *fc = f + d.link_off;
+        switch (mu) {
+        case 0: acc = metro_link<S, 0>(fc, d.g, s, q); break;
+        case 1: acc = metro_link<S, 1>(fc, d.g, s, q); break;
+        case 2: acc = metro_link<S, 2>(fc, d.g, s, q); break;
+        default: acc = metro_link<S, 3>(fc, d.g, s, q); break;
+        }
+    }
+    if (acc) atomicAdd(accepted + c, (unsigned long long)acc);
+}
+
+}  // namespace lym
+
+using lym::ConfDesc;
+using lym::ksum;
+
+struct lym_batch {
+    int device = 0, prec = LYM_PREC_FP64, nconf = 0;
+    std::vector<ConfDesc> desc;
+    std::vector<std::array<int, 4>> dims;
+    ConfDesc *d_desc = nullptr;
+    long total_links = 0, total_sites = 0, ntiles = 0, nhalf = 0;
+    hipStream_t stream = nullptr;
+    void *W[2] = {nullptr, nullptr};
+    int cur = 0;
+    void *acc = nullptr;
+    ksum *partials = nullptr;
+    double *obs = nullptr;
+    int obs_cap = 0;
+    double *staging = nullptr;
+    long staging_sites = 0;
+    unsigned long long *accepted = nullptr;
+    int grid = 0, sched = lym::SCHED_XCD;  // launch shape of the stage kernel
+    std::string tuned;
+    hipGraphExec_t graph = nullptr;
+    std::string graph_key;
+    int graph_parity_flip = 0;
+    std::string err;
+};
+
+namespace {
+
+int bfail(lym_batch *b, int code, const std::string &m)
+{
+    if (b) b->err = m;
+    return code;
+}
+int bhip(lym_batch *b, hipError_t e, const char *what)
+{
+    return bfail(b, e == hipErrorOutOfMemory ? LYM_ERR_NOMEM : LYM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define BHIP(b, call)                                    \
+    do {                                                 \
+        hipError_t e_ = (call);                          \
+        if (e_ != hipSuccess) return bhip(b, e_, #call); \
+    } while (0)
+#define BLAUNCHED(b, what)                             \
+    do {                                               \
+        hipError_t e_ = hipGetLastError();             \
+        if (e_ != hipSuccess) return bhip(b, e_, what); \
+    } while (0)
+
+unsigned nblocks(long n, int per) { return unsigned((n + per - 1) / per); }
+
+template <typename S> int enqueue_step(lym_batch *b, double eps)
+{
+    using F2 = lym::F2<S>;
+    F2 *x = static_cast<F2 *>(b->W[b->cur]);
+    F2 *y = static_cast<F2 *>(b->W[1 - b->cur]);
+    S *acc = static_cast<S *>(b->acc);
+    constexpr bool re = sizeof(S) == 4;
+    const unsigned g = b->grid > 0 ? unsigned(b->grid) : unsigned(b->ntiles);
+    lym::k_batch_flow_stage<S, 1, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
+    lym::k_batch_flow_stage<S, 2, re><<<g, 256, 0, b->stream>>>(y, x, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
+    lym::k_batch_flow_stage<S, 3, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
+    BLAUNCHED(b, "k_batch_flow_stage");
+    b->cur = 1 - b->cur;
+    return LYM_OK;
+}
+int step(lym_batch *b, double eps)
+{
+    return b->prec == LYM_PREC_FP64 ? enqueue_step<double>(b, eps) : enqueue_step<float>(b, eps);
+}
+
+template <typename S> int enqueue_obs(lym_batch *b, int slot)
+{
+    const unsigned g = unsigned(std::min<long>(b->ntiles, 65535L * 4));
+    lym::k_batch_observables<S><<<g, 192, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]), b->d_desc,
+                                                          b->nconf, b->ntiles, b->partials);
+    lym::k_batch_reduce<<<b->nconf, 256, 0, b->stream>>>(b->partials, b->d_desc, b->nconf, b->ntiles,
+                                                        b->obs + size_t(4) * b->nconf * slot);
+    BLAUNCHED(b, "k_batch_observables");
+    return LYM_OK;
+}
+int observables(lym_batch *b, int slot)
+{
+    return b->prec == LYM_PREC_FP64 ? enqueue_obs<double>(b, slot) : enqueue_obs<float>(b, slot);
+}
+
+int ensure_slots(lym_batch *b, int n)
+{
+    if (n <= b->obs_cap) return LYM_OK;
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->obs) hipFree(b->obs);
+    b->obs = nullptr;
+    BHIP(b, hipMalloc(&b->obs, sizeof(double) * 4 * size_t(b->nconf) * n));
+    b->obs_cap = n;
+    if (b->graph) {  // the captured graph points at the old buffer
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graph_key.clear();
+    }
+    return LYM_OK;
+}
+
+void normalise(const lym_batch *b, int c, const double *raw, double out[4])
+{
+    const double qn = 16 * 16 * M_PI * M_PI;
+    out[0] = raw[0] / 18.0 / double(b->desc[c].g.V);
+    out[1] = raw[1];
+    out[2] = raw[2] / qn;
+    out[3] = raw[3] / qn;
+}
+
+// enqueue the whole flow (rows-1 steps, measurements into slots)
+int enqueue_flow(lym_batch *b, double eps, int rows, int every)
+{
+    int slot = 0, rc;
+    if ((rc = observables(b, slot++))) return rc;
+    for (int t = 1; t < rows; t++) {
+        if ((rc = step(b, eps))) return rc;
+        if (t % every == 0)
+            if ((rc = observables(b, slot++))) return rc;
+    }
+    return LYM_OK;
+}
+
+std::string shape_key(const lym_batch *b)
+{
+    std::ostringstream k;
+    k << "gfx950|prec" << b->prec;
+    for (auto &d : b->dims) k << "|" << d[0] << "x" << d[1] << "x" << d[2] << "x" << d[3];
+    return k.str();
+}
+
+}  // namespace
+
+extern "C" {
+
+int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision, int device)
+{
+    if (!out || !dims || nconf < 1 || nconf > lym::kMaxConf) return LYM_ERR_ARG;
+    *out = nullptr;
+    if (precision != LYM_PREC_FP64 && precision != LYM_PREC_MIXED) return LYM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
+    if (device < 0 || device >= ndev) return LYM_ERR_ARG;
+    lym_batch *b = new lym_batch;
+    b->device = device;
+    b->prec = precision;
+    b->nconf = nconf;
+    long links = 0, tiles = 0, half = 0, maxV = 0;
+    for (int c = 0; c < nconf; c++) {
+        ConfDesc d{};
+        long V = 1, st = 1;
+        for (int k = 0; k < 4; k++) {
+            const int L = dims[4 * c + k];
+            if (L < 2) {
+                delete b;
+                return LYM_ERR_ARG;
+            }
+            d.g.L[k] = L;
+            d.g.stride[k] = int(st);
+            st *= L;
+            V *= L;
+        }
+        if (4 * V >= (1L << 31)) {
+            delete b;
+            return LYM_ERR_ARG;
+        }
+        d.g.V = V;
+        d.link_off = links * 9;
+        d.acc_off = links * 8;
+        d.tile0 = tiles;
+        d.half0 = half;
+        links += 4 * V;
+        tiles += (V + lym::kSitesPerBlock - 1) / lym::kSitesPerBlock;
+        half += V / 2;
+        maxV = std::max(maxV, V);
+        b->desc.push_back(d);
+        b->dims.push_back({dims[4 * c], dims[4 * c + 1], dims[4 * c + 2], dims[4 * c + 3]});
+    }
+    b->total_links = links;
+    b->total_sites = links / 4;
+    b->ntiles = tiles;
+    b->nhalf = half;
+    b->staging_sites = maxV;
+    auto bail = [&](int code) {
+        lym_batch_destroy(b);
+        return code;
+    };
+    const size_t rs = precision == LYM_PREC_FP64 ? 8 : 4;
+    if (hipSetDevice(device) != hipSuccess) return bail(LYM_ERR_HIP);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(LYM_ERR_HIP);
+    for (int k = 0; k < 2; k++)
+        if (hipMalloc(&b->W[k], rs * 18 * size_t(links)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->acc, rs * 8 * size_t(links)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->partials, sizeof(ksum) * 4 * size_t(tiles)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->staging, sizeof(double) * 18 * size_t(maxV)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->accepted, sizeof(unsigned long long) * nconf) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->d_desc, sizeof(ConfDesc) * nconf) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(ConfDesc) * nconf, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(LYM_ERR_HIP);
+    if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
+    // cold start
+    for (int c = 0; c < nconf; c++) {
+        const long V = b->desc[c].g.V;
+        if (precision == LYM_PREC_FP64)
+            lym::k_cold<double><<<nblocks(4 * V, 256), 256, 0, b->stream>>>(
+                static_cast<double2 *>(b->W[0]) + b->desc[c].link_off, V);
+        else
+            lym::k_cold<float><<<nblocks(4 * V, 256), 256, 0, b->stream>>>(
+                static_cast<float2 *>(b->W[0]) + b->desc[c].link_off, V);
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(LYM_ERR_HIP);
+    *out = b;
+    return LYM_OK;
+}
+
+int lym_batch_destroy(lym_batch *b)
+{
+    if (!b) return LYM_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    for (void *p : {b->W[0], b->W[1], b->acc, (void *)b->partials, (void *)b->obs, (void *)b->staging,
+                    (void *)b->accepted, (void *)b->d_desc})
+        if (p) hipFree(p);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return LYM_OK;
+}
+
+const char *lym_batch_last_error(const lym_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *sched)
+{
+    if (!b) return LYM_ERR_ARG;
+    if (nconf) *nconf = b->nconf;
+    if (total_sites) *total_sites = b->total_sites;
+    if (grid) *grid = b->grid;
+    if (sched) *sched = b->sched;
+    return LYM_OK;
+}
+
+static int check_conf(lym_batch *b, int c)
+{
+    if (c < 0 || c >= b->nconf) return bfail(b, LYM_ERR_ARG, "configuration index out of range");
+    return LYM_OK;
+}
+
+int lym_batch_upload_links(lym_batch *b, int c, const double *const by_mu[4])
+{
+    if (!b || !by_mu) return LYM_ERR_ARG;
+    if (check_conf(b, c)) return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    const ConfDesc &d = b->desc[c];
+    const long V = d.g.V;
+    for (int mu = 0; mu < 4; mu++) {
+        if (!by_mu[mu]) return bfail(b, LYM_ERR_ARG, "null link array");
+        BHIP(b, hipMemcpyAsync(b->staging, by_mu[mu], sizeof(double) * 18 * V, hipMemcpyHostToDevice, b->stream));
+        if (b->prec == LYM_PREC_FP64)
+            lym::k_aos_to_soa<double><<<nblocks(V, 256), 256, 0, b->stream>>>(
+                b->staging, static_cast<double2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, V);
+        else
+            lym::k_aos_to_soa<float><<<nblocks(V, 256), 256, 0, b->stream>>>(
+                b->staging, static_cast<float2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, V);
+        BLAUNCHED(b, "k_aos_to_soa");
+    }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return LYM_OK;
+}
+
+int lym_batch_download_links(lym_batch *b, int c, double *const by_mu[4])
+{
+    if (!b || !by_mu) return LYM_ERR_ARG;
+    if (check_conf(b, c)) return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    const ConfDesc &d = b->desc[c];
+    const long V = d.g.V;
+    for (int mu = 0; mu < 4; mu++) {
+        if (!by_mu[mu]) return bfail(b, LYM_ERR_ARG, "null link array");
+        if (b->prec == LYM_PREC_FP64)
+            lym::k_soa_to_aos<double><<<nblocks(V, 256), 256, 0, b->stream>>>(
+                static_cast<const double2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, b->staging, V);
+        else
+            lym::k_soa_to_aos<float><<<nblocks(V, 256), 256, 0, b->stream>>>(
+                static_cast<const float2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, b->staging, V);
+        BLAUNCHED(b, "k_soa_to_aos");
+        BHIP(b, hipMemcpyAsync(by_mu[mu], b->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, b->stream));
+        BHIP(b, hipStreamSynchronize(b->stream));
+    }
+    return LYM_OK;
+}
+
+int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds)
+{
+    if (!b || !seeds) return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    for (int c = 0; c < b->nconf; c++) {
+        const long V = b->desc[c].g.V;
+        if (b->prec == LYM_PREC_FP64)
+            lym::k_hot<double><<<nblocks(4 * V, 256), 256, 0, b->stream>>>(
+                static_cast<double2 *>(b->W[b->cur]) + b->desc[c].link_off, V, seeds[c]);
+        else
+            lym::k_hot<float><<<nblocks(4 * V, 256), 256, 0, b->stream>>>(
+                static_cast<float2 *>(b->W[b->cur]) + b->desc[c].link_off, V, seeds[c]);
+        BLAUNCHED(b, "k_hot");
+    }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return LYM_OK;
+}
+
+int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,
+                         const uint64_t *seeds, int64_t *accepted)
+{
+    if (!b || !seeds) return LYM_ERR_ARG;
+    if (nhit < 1 || nsweeps < 0 || first_sweep < 0 || (candidate != LYM_CAND_SU2 && candidate != LYM_CAND_EXPH) ||
+        !(eps > 0 && eps < 1))
+        return bfail(b, LYM_ERR_ARG, "bad Metropolis arguments");
+    for (auto &d : b->dims)
+        for (int k = 0; k < 4; k++)
+            if (d[k] % 2) return bfail(b, LYM_ERR_ARG, "checkerboard needs even extents");
+    BHIP(b, hipSetDevice(b->device));
+    for (int c = 0; c < b->nconf; c++) b->desc[c].seed = seeds[c];
+    BHIP(b, hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(ConfDesc) * b->nconf, hipMemcpyHostToDevice, b->stream));
+    lym::k_zero_u64<<<nblocks(b->nconf, 64), 64, 0, b->stream>>>(b->accepted, b->nconf);
+    const int nu = candidate == LYM_CAND_SU2 ? 13 : 9;
+    const unsigned nb = nblocks(b->nhalf, 256);
+    for (int sw = 0; sw < nsweeps; sw++) {
+        lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, 0, nullptr};
+        for (int mu = 0; mu < 4; mu++)
+            for (int par = 0; par < 2; par++) {
+                p.parity = par;
+                if (b->prec == LYM_PREC_FP64)
+                    lym::k_batch_metropolis<double><<<nb, 256, 0, b->stream>>>(
+                        static_cast<double2 *>(b->W[b->cur]), b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+                else
+                    lym::k_batch_metropolis<float><<<nb, 256, 0, b->stream>>>(
+                        static_cast<float2 *>(b->W[b->cur]), b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+                BLAUNCHED(b, "k_batch_metropolis");
+            }
+    }
+    std::vector<unsigned long long> acc(b->nconf);
+    BHIP(b, hipMemcpyAsync(acc.data(), b->accepted, sizeof(unsigned long long) * b->nconf, hipMemcpyDeviceToHost,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    if (accepted)
+        for (int c = 0; c < b->nconf; c++) accepted[c] = int64_t(acc[c]);
+    return LYM_OK;
+}
+
+int lym_batch_observables(lym_batch *b, double *out)
+{
+    if (!b || !out) return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    int rc = observables(b, 0);
+    if (rc) return rc;
+    std::vector<double> raw(4 * size_t(b->nconf));
+    BHIP(b, hipMemcpyAsync(raw.data(), b->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int c = 0; c < b->nconf; c++) normalise(b, c, &raw[4 * size_t(c)], out + 4 * size_t(c));
+    return LYM_OK;
+}
+
+int lym_batch_set_schedule(lym_batch *b, int grid, int sched)
+{
+    if (!b || grid < 0 || (sched != lym::SCHED_LINEAR && sched != lym::SCHED_XCD)) return LYM_ERR_ARG;
+    b->grid = grid;
+    b->sched = sched;
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graph_key.clear();
+    }
+    return LYM_OK;
+}
+
+int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
+                   int max_rows, int *rows_out)
+{
+    if (!b || !series || measure_every < 1 || !(eps > 0) || !(tau > 0))
+        return b ? bfail(b, LYM_ERR_ARG, "bad flow arguments") : LYM_ERR_ARG;
+    const int rows = int(tau / eps);  // src/Apps/wilsonflow.cpp:117
+    if (rows_out) *rows_out = rows;
+    if (rows < 1 || rows > max_rows) return bfail(b, LYM_ERR_ARG, "series too small for int(tau/eps) rows");
+    BHIP(b, hipSetDevice(b->device));
+    const int nmeas = 1 + (rows - 1) / measure_every;
+    int rc = ensure_slots(b, nmeas);
+    if (rc) return rc;
+    const int nsteps = rows - 1;
+    if (use_graph) {
+        // the graph is recorded for fixed buffers: W[cur] -> ... ; the final
+        // field lands in W[cur ^ (nsteps & 1)], so one graph per start parity
+        std::ostringstream key;
+        key << eps << "|" << rows << "|" << measure_every << "|" << b->cur << "|" << b->grid << "|" << b->sched;
+        if (!b->graph || b->graph_key != key.str()) {
+            if (b->graph) hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            const int cur0 = b->cur;
+            BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_flow(b, eps, rows, measure_every);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            b->cur = cur0;
+            if (rc) {
+                if (g) hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return bhip(b, e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) return bhip(b, e, "hipGraphInstantiate");
+            b->graph_key = key.str();
+        }
+        BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        if (nsteps & 1) b->cur = 1 - b->cur;
+    } else {
+        if ((rc = enqueue_flow(b, eps, rows, measure_every))) return rc;
+    }
+    std::vector<double> raw(size_t(4) * b->nconf * nmeas);
+    BHIP(b, hipMemcpyAsync(raw.data(), b->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    for (int c = 0; c < b->nconf; c++) {
+        double tcur = 0.0;
+        int slot = 0;
+        for (int t = 0; t < rows; t++) {
+            if (t > 0) tcur += eps;
+            double *row = series + (size_t(c) * max_rows + t) * 4;
+            row[0] = tcur;
+            if (t % measure_every == 0) {
+                double o[4];
+                normalise(b, c, &raw[(size_t(slot) * b->nconf + c) * 4], o);
+                row[1] = o[0];
+                row[2] = o[2];
+                row[3] = o[1];
+                slot++;
+            } else {
+                row[1] = row[2] = row[3] = NAN;
+            }
+        }
+    }
+    return LYM_OK;
+}
+
+// Times candidate launch shapes of the RK stage kernel on the batch itself
+// (stage 1 reads W[cur] and writes W[1-cur] and the accumulator, so the field
+// is left unchanged) and keeps the fastest.  Candidates: schedule LINEAR or
+// XCD x grid = one workgroup per tile, or a persistent grid of
+// {1, 2, 4, 8, 16} x 256 workgroups.  Results are cached in `cache_path`
+// (one line per batch shape) when it is non-null.
+int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len)
+{
+    if (!b) return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    const std::string key = shape_key(b);
+    std::map<std::string, std::string> cache;
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            auto p = line.find('\t');
+            if (p != std::string::npos) cache[line.substr(0, p)] = line.substr(p + 1);
+        }
+    }
+    int best_grid = 0, best_sched = lym::SCHED_XCD;
+    double best_ms = -1;
+    auto it = cache.find(key);
+    if (it != cache.end() && sscanf(it->second.c_str(), "%d %d %lf", &best_grid, &best_sched, &best_ms) == 3) {
+        b->tuned = "cached";
+    } else {
+        std::vector<std::pair<int, int>> cands;
+        for (int sched : {lym::SCHED_LINEAR, lym::SCHED_XCD}) {
+            cands.push_back({0, sched});
+            for (int m : {1, 2, 4, 8, 16})
+                if (256L * m < b->ntiles) cands.push_back({256 * m, sched});
+        }
+        hipEvent_t e0, e1;
+        BHIP(b, hipEventCreate(&e0));
+        BHIP(b, hipEventCreate(&e1));
+        for (auto &cd : cands) {
+            const unsigned g = cd.first > 0 ? unsigned(cd.first) : unsigned(b->ntiles);
+            float best_c = 1e30f;
+            for (int rep = 0; rep < 4; rep++) {
+                BHIP(b, hipEventRecord(e0, b->stream));
+                if (b->prec == LYM_PREC_FP64)
+                    lym::k_batch_flow_stage<double, 1, false><<<g, 256, 0, b->stream>>>(
+                        static_cast<double2 *>(b->W[b->cur]), static_cast<double2 *>(b->W[1 - b->cur]),
+                        static_cast<double *>(b->acc), b->d_desc, b->nconf, b->ntiles, cd.second, 0.0);
+                else
+                    lym::k_batch_flow_stage<float, 1, true><<<g, 256, 0, b->stream>>>(
+                        static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
+                        static_cast<float *>(b->acc), b->d_desc, b->nconf, b->ntiles, cd.second, 0.0);
+                BLAUNCHED(b, "k_batch_flow_stage");
+                BHIP(b, hipEventRecord(e1, b->stream));
+                BHIP(b, hipEventSynchronize(e1));
+                float ms = 0;
+                BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) best_c = std::min(best_c, ms);  // first run warms caches
+            }
+            if (best_ms < 0 || best_c < best_ms) {
+                best_ms = best_c;
+                best_grid = cd.first;
+                best_sched = cd.second;
+            }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        b->tuned = "measured";
+        if (cache_path) {
+            std::ofstream o(cache_path, std::ios::app);
+            o << key << "\t" << best_grid << " " << best_sched << " " << best_ms << "\n";
+        }
+    }
+    lym_batch_set_schedule(b, best_grid, best_sched);
+    if (summary && summary_len > 0) {
+        snprintf(summary, size_t(summary_len), "grid=%d sched=%s stage1_ms=%.4f source=%s tiles=%ld", best_grid,
+                 best_sched == lym::SCHED_XCD ? "xcd" : "linear", best_ms, b->tuned.c_str(), b->ntiles);
+    }
+    return LYM_OK;
+}
+
+}  // extern "C"

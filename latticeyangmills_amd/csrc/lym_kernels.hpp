@@ -247,7 +247,7 @@ __global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f
 }
 
 // Sum nblk block partials into dst[0..3] (one workgroup).
-__global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__restrict__ partials, int nblk,
+static __global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__restrict__ partials, int nblk,
                                                          double *__restrict__ dst)
 {
     ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__restrict_
 // ------------------------------------------------ layout and start fields ---
 
 // clears counters with a kernel (a memset node misbehaves on graph replays)
-__global__ void k_zero_u64(unsigned long long *p, int n)
+static __global__ void k_zero_u64(unsigned long long *p, int n)
 {
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }

@@ -1,0 +1,30 @@
+"""The C-ABI library loads on a machine without a GPU and exports every
+function that include/lym.h declares (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "latticeyangmills_amd", "liblym.so")
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lym_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h")))
+def test_exports_every_declared_symbol(header):
+    lib = ctypes.CDLL(LIB)
+    names = declared(header)
+    assert names, header
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_package_binding_loads():
+    from latticeyangmills_amd import lib
+    assert lib() is not None

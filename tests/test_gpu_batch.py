@@ -1,0 +1,115 @@
+"""Batched ensemble path (include/lym_batch.h): each lattice of an uneven
+batch must give what the single-lattice path gives on it, whatever the launch
+schedule, with and without the HIP graph; graph replays are checked against
+eager runs with eager work between them."""
+import numpy as np
+import pytest
+
+from latticeyangmills_amd import LatticeContext
+from latticeyangmills_amd.batch import BatchContext
+from tests import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(4, 4, 4, 4), (6, 4, 4, 2), (8, 4, 4, 4), (4, 4, 6, 6)]
+
+
+@pytest.fixture(scope="module")
+def inputs():
+    return [orc.hot_start(d, 100 + i) for i, d in enumerate(DIMS)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_upload_download_and_observables(inputs, precision):
+    with BatchContext(DIMS, precision) as b:
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        obs = b.observables()
+        for c, L in enumerate(inputs):
+            got = b.download(c)
+            if precision == "fp64":
+                assert np.array_equal(got, L)
+            want = orc.observables(L, DIMS[c])
+            tol = 1e-12 if precision == "fp64" else 1e-6
+            assert abs(obs[c, 0] / want[0] - 1) < tol
+            assert abs(obs[c, 1] / want[1] - 1) < tol
+            assert abs(obs[c, 2] - want[2]) < tol * want[3]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("grid,sched", [(0, "linear"), (0, "xcd"), (8, "xcd"), (3, "linear")])
+def test_flow_matches_single_lattice_path(inputs, precision, grid, sched):
+    eps, tau, every = 0.02, 0.2, 3
+    singles = []
+    for c, L in enumerate(inputs):
+        with LatticeContext(DIMS[c], precision) as ctx:
+            ctx.upload(L)
+            s = ctx.flow(eps, tau, every)
+            singles.append((s, ctx.download()))
+    with BatchContext(DIMS, precision) as b:
+        b.set_schedule(grid, sched)
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        series = b.flow(eps, tau, every, use_graph=False)
+        for c in range(len(DIMS)):
+            s, links = singles[c]
+            assert np.array_equal(b.download(c), links)
+            m = ~np.isnan(s[:, 1])
+            assert np.array_equal(np.isnan(series[c, :, 1]), ~m)
+            assert np.allclose(series[c][m], s[m], rtol=1e-13, atol=0)
+
+
+def test_graph_replays_match_eager(inputs):
+    eps, tau, every = 0.02, 0.16, 2  # 7 steps: odd, so the start buffer alternates
+    with BatchContext(DIMS, "mixed") as b:
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        eager = b.flow(eps, tau, every, use_graph=False)
+        eager_links = [b.download(c) for c in range(len(DIMS))]
+        for rep in range(4):
+            for c, L in enumerate(inputs):
+                b.upload(c, L)
+            b.observables()  # eager work between replays
+            got = b.flow(eps, tau, every, use_graph=True)
+            assert np.array_equal(np.nan_to_num(got, nan=7.0), np.nan_to_num(eager, nan=7.0)), rep
+            for c in range(len(DIMS)):
+                assert np.array_equal(b.download(c), eager_links[c]), (rep, c)
+
+
+def test_metropolis_matches_single_lattice_path(inputs):
+    seeds = [11, 22, 33, 44]
+    with BatchContext(DIMS, "fp64") as b:
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        acc = b.metropolis(6.0, 0.24, 4, 2, seeds, first_sweep=3)
+        for c, L in enumerate(inputs):
+            with LatticeContext(DIMS[c], "fp64") as ctx:
+                ctx.upload(L)
+                a1 = ctx.metropolis(6.0, 0.24, 4, 2, 3, "exph", seeds[c])
+                assert a1 == acc[c]
+                assert np.array_equal(ctx.download(), b.download(c))
+
+
+def test_hot_start_matches_single(inputs):
+    seeds = [5, 6, 7, 8]
+    with BatchContext(DIMS, "fp64") as b:
+        b.set_hot(seeds)
+        for c in range(len(DIMS)):
+            with LatticeContext(DIMS[c], "fp64") as ctx:
+                ctx.set_hot(seeds[c])
+                assert np.array_equal(ctx.download(), b.download(c))
+
+
+def test_autotune_keeps_results(inputs, tmp_path):
+    cache = str(tmp_path / "tune.txt")
+    with BatchContext(DIMS, "mixed") as b:
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        ref = b.flow(0.02, 0.1, 1, use_graph=False)
+        summary = b.autotune(cache)
+        assert "grid=" in summary and "measured" in summary
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        got = b.flow(0.02, 0.1, 1, use_graph=True)
+        assert np.allclose(got, ref, rtol=1e-13, atol=0)
+        assert "cached" in b.autotune(cache)

@@ -1,0 +1,182 @@
+"""Parity of the HIP path (through the C ABI of include/lym.h) against the CPU
+oracle on identical inputs.
+
+Tolerances: FP64 path 1e-12 relative on plaquette and energy at every flow
+time; Q to 1e-12 of its cancellation scale (sum of |terms|), because Q sums
+terms that cancel heavily.  Metropolis in test mode consumes the same
+pre-drawn uniforms as the oracle and must reproduce it link by link.
+"""
+import numpy as np
+import pytest
+
+from latticeyangmills_amd import LatticeContext
+from tests import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(4, 4, 4, 4), (4, 6, 8, 2), (6, 4, 4, 8)]
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def assert_obs_close(got, want, tol=1e-12):
+    plaq, energy, topc, scale = want
+    assert rel(got[0], plaq) < tol, (got, want)
+    assert rel(got[1], energy) < tol, (got, want)
+    assert abs(got[2] - topc) <= tol * max(scale, 1e-300) + 1e-300, (got, want)
+
+
+@pytest.fixture(scope="module")
+def therm8():
+    dims = (8, 8, 8, 8)
+    return dims, orc.thermalised(dims, beta=6.0, sweeps=8, seed=11)
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_upload_download_roundtrip(dims):
+    links = orc.hot_start(dims, 3)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        assert np.array_equal(ctx.download(), links)
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_cold_start(dims):
+    with LatticeContext(dims) as ctx:
+        plaq, energy, topc, scale = ctx.observables()
+        assert plaq == pytest.approx(1.0, abs=1e-15)
+        assert abs(energy) < 1e-20 and abs(topc) < 1e-20
+        assert np.array_equal(ctx.download(), orc.cold_start(dims))
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_hot_start_matches_oracle(dims):
+    want = orc.hot_start(dims, 1234)
+    with LatticeContext(dims) as ctx:
+        ctx.set_hot(1234)
+        got = ctx.download()
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("dims", DIMS)
+def test_observables_hot(dims):
+    links = orc.hot_start(dims, 5)
+    want = orc.observables(links, dims, 0)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        got = ctx.observables()
+    assert_obs_close(got, want)
+    assert rel(got[3], want[3]) < 1e-12
+
+
+def test_observables_thermalised(therm8):
+    dims, links = therm8
+    want = orc.observables(links, dims, 0)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        got = ctx.observables()
+    assert_obs_close(got, want)
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("mu", [0, 1, 2, 3])
+def test_staples_and_derivative(dims, mu):
+    links = orc.hot_start(dims, 17)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        om, z = ctx.staples(mu), ctx.derivative(mu)
+    om_ref, z_ref = orc.staples(links, dims, mu), orc.derivative(links, dims, mu)
+    assert np.max(np.abs(om - om_ref)) < 1e-12 * np.max(np.abs(om_ref))
+    assert np.max(np.abs(z - z_ref)) < 1e-12 * np.max(np.abs(z_ref))
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 8, 2)])
+def test_one_flow_step(dims):
+    links = orc.hot_start(dims, 23)
+    want = orc.flow_step(links, dims, 0.02)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        got = ctx.download()
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+def test_flow_series_thermalised(therm8):
+    """WilsonFlow::applyWilsonFlow on 8^4, eps 0.01 to t = 0.2, every row."""
+    dims, links = therm8
+    want, want_links = orc.flow(links, dims, 0.01, 0.2, 1, 0)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        got = ctx.flow(0.01, 0.2, 1)
+        got_links = ctx.download()
+    assert got.shape == want.shape == (20, 4)
+    assert np.array_equal(got[:, 0], want[:, 0])
+    for r in range(want.shape[0]):
+        assert rel(got[r, 1], want[r, 1]) < 1e-12, r
+        assert rel(got[r, 3], want[r, 3]) < 1e-12, r
+    assert np.max(np.abs(got_links - want_links)) < 1e-12
+
+
+def test_flow_measure_every(therm8):
+    dims, links = therm8
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        every = ctx.flow(0.02, 0.3, 5)
+        ctx.upload(links)
+        all_rows = ctx.flow(0.02, 0.3, 1)
+    assert every.shape == all_rows.shape == (15, 4)
+    for r in range(15):
+        if r % 5 == 0:
+            assert np.array_equal(every[r], all_rows[r])
+        else:
+            assert np.all(np.isnan(every[r, 1:]))
+
+
+@pytest.mark.parametrize("candidate", ["su2", "exph"])
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (2, 4, 6, 4)])
+def test_metropolis_predrawn_per_link(dims, candidate):
+    """Test mode: identical pre-drawn uniforms give identical links."""
+    V = orc.volume(dims)
+    nhit, nsweeps = 5, 2
+    nu = 13 if candidate == "su2" else 9
+    rng = np.random.default_rng(99)
+    u = rng.random(nsweeps * 4 * V * nhit * nu)
+    links = orc.hot_start(dims, 8)
+    cid = 0 if candidate == "su2" else 1
+    want, acc_want = orc.metropolis(links, dims, 6.0, 0.24, nhit, nsweeps, 0, cid, 0, u)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        acc = ctx.metropolis(6.0, 0.24, nhit, nsweeps, 0, candidate, 0, u)
+        got = ctx.download()
+    assert acc == acc_want
+    assert np.max(np.abs(got - want)) < 1e-12
+
+
+@pytest.mark.parametrize("candidate", ["su2", "exph"])
+def test_metropolis_philox_per_link(candidate):
+    """Production mode: the oracle draws the same Philox stream."""
+    dims = (4, 4, 4, 4)
+    links = orc.hot_start(dims, 4)
+    cid = 0 if candidate == "su2" else 1
+    want, acc_want = orc.metropolis(links, dims, 6.0, 0.24, 10, 3, 7, cid, 2024)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        acc = ctx.metropolis(6.0, 0.24, 10, 3, 7, candidate, 2024)
+        got = ctx.download()
+    assert acc == acc_want
+    assert np.max(np.abs(got - want)) < 1e-11
+
+
+def test_mixed_precision_against_fp64(therm8):
+    """FP32 links/staples/clover, FP64 exp and reductions: error stated vs FP64."""
+    dims, links = therm8
+    want, _ = orc.flow(links, dims, 0.02, 0.5, 5, 1)
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.upload(links)
+        got = ctx.flow(0.02, 0.5, 5)
+    for r in range(0, want.shape[0], 5):
+        assert rel(got[r, 1], want[r, 1]) < 1e-6, r
+        assert rel(got[r, 3], want[r, 3]) < 1e-5, r
+        assert abs(got[r, 2] - want[r, 2]) < 1e-4, r

@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--workload", default="mixed-32x64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--sweeps", type=int, default=None, help="override the Metropolis sweep count")
+    ap.add_argument("--calibrate", action="store_true", help="also time a 1 GiB streaming copy (HBM probe)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -131,11 +133,12 @@ def main():
     # thermalise in FP64 on the GPU
     t = time.perf_counter()
     with LatticeContext(dims, "fp64") as th:
-        acc = th.metropolis(w["beta"], 0.24, 10, w["sweeps"], 0, "exph", w["seed"])
+        sweeps = args.sweeps if args.sweeps is not None else w["sweeps"]
+        acc = th.metropolis(w["beta"], 0.24, 10, sweeps, 0, "exph", w["seed"])
         plaq0 = th.observables()[0]
         links = th.download()
-    log(f"[bench] thermalised: {w['sweeps']} sweeps in {time.perf_counter() - t:.1f} s, plaq {plaq0:.6f}, "
-        f"acceptance {acc / (w['sweeps'] * 4 * V * 10):.3f}")
+    log(f"[bench] thermalised: {sweeps} sweeps in {time.perf_counter() - t:.1f} s, plaq {plaq0:.6f}, "
+        f"acceptance {acc / max(sweeps * 4 * V * 10, 1):.3f}")
 
     results = {}
     for prec in ([w["precision"], "fp64"] if w["precision"] != "fp64" else ["fp64"]):
@@ -165,7 +168,7 @@ def main():
         "vs_baseline": None, "dtype": "f32+f64" if main_prec == "mixed" else "f64",
         "data": "synthetic: Philox-seeded checkerboard Metropolis thermalised on the GPU",
         "config": {"workload": args.workload, "lattice": "x".join(map(str, dims)), "beta": w["beta"],
-                   "metropolis_sweeps": w["sweeps"], "seed": w["seed"], "flow_eps": eps, "flow_t_max": tau - eps,
+                   "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "flow_t_max": tau - eps,
                    "measure_every": every, "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
@@ -190,6 +193,9 @@ def main():
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),
             "t2E_at_tmax": s64[last, 0] ** 2 * e_mean(s64[last, 3], V), "Q_at_tmax": s64[last, 2],
         }
+    if args.calibrate:
+        from latticeyangmills_amd.capi import copy_bandwidth
+        out["hbm_copy_gbs"] = copy_bandwidth(1 << 30, 10)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w["beta"], w["seed"])
     print(json.dumps(out), flush=True)

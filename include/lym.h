@@ -41,7 +41,8 @@ extern "C" {
 typedef struct lym_ctx lym_ctx;
 
 /* Creates a context for one Lx*Ly*Lz*Lt periodic lattice on GPU `device`;
- * links start cold (all 1).  Replaces WilsonFlow::createLattice /
+ * links start cold (all 1).  Extents >= 2; one direction of the field must
+ * stay below 2 GiB (V < 14.9M sites in FP64, 29.8M in mixed precision).  Replaces WilsonFlow::createLattice /
  * App::createLattice (src/Apps/wilsonflow.cpp:87-91, src/Apps/app.cpp:41-44). */
 int lym_create(lym_ctx **ctx, const int dims[4], int precision, int device);
 int lym_destroy(lym_ctx *ctx);
@@ -107,6 +108,12 @@ int lym_synchronize(lym_ctx *ctx);
  * time of the whole sequence on the stream. */
 int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, double *total_ms,
                      double stage_ms[4]);
+
+/* Measurement helper with no counterpart in the reference: a streaming
+ * device-to-device copy of `bytes` bytes repeated `reps` times on GPU
+ * `device`; *gbs = (read + written bytes) / mean kernel time.  Used to confirm
+ * the HBM bandwidth of the box and to calibrate HBM byte counters. */
+int lym_copy_bandwidth(int device, int64_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

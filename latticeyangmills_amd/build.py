@@ -37,6 +37,26 @@ def build_lym(extra=()):
     return out
 
 
+HOST = os.path.join(PKG, "host")
+HOST_SOURCES = ["math.cpp", "gpuclasses.cpp", "confio.cpp"]
+CXX = os.environ.get("CXX", "g++")
+CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-result"]
+
+
+def build_host():
+    """liblymhost.so (the C++ classes over the C ABI) and the lym_app CLI."""
+    inc = ["-I", INCLUDE, "-I", os.path.join(HOST, "include")]
+    link = ["-L", PKG, "-llym", "-Wl,-rpath,$ORIGIN"]
+    out = os.path.join(PKG, "liblymhost.so")
+    srcs = [os.path.join(HOST, "src", s) for s in HOST_SOURCES]
+    _run([CXX, *CXX_FLAGS, *inc, "-shared", "-o", out + ".tmp", *srcs, *link])
+    os.replace(out + ".tmp", out)
+    app = os.path.join(PKG, "lym_app")
+    _run([CXX, *CXX_FLAGS, *inc, "-o", app + ".tmp", os.path.join(HOST, "src", "lym_app.cpp"),
+          "-L", PKG, "-llymhost", *link])
+    os.replace(app + ".tmp", app)
+
+
 def build_oracle(with_ref=None):
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
     ref = os.environ.get("LYM_REFERENCE", "/root/reference")
@@ -48,10 +68,12 @@ def build_oracle(with_ref=None):
 
 def build_all():
     build_lym()
+    build_host()
     build_oracle()
 
 
 if __name__ == "__main__":
     extra = sys.argv[1:]
     build_lym(extra)
+    build_host()
     build_oracle()

@@ -49,7 +49,17 @@ def _sig(lib):
                                    c.POINTER(c.c_int64)]
     lib.lym_synchronize.argtypes = [_P]
     lib.lym_profile_flow.argtypes = [_P, c.c_double, _I, _I, _D, _D]
+    lib.lym_copy_bandwidth.argtypes = [_I, c.c_int64, _I, _D]
     return lib
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=20, device=0):
+    """GB/s of a streaming device copy (read + write bytes over kernel time)."""
+    gbs = ctypes.c_double(0)
+    rc = lib().lym_copy_bandwidth(device, nbytes, reps, ctypes.byref(gbs))
+    if rc != LYM_OK:
+        raise LymError(rc, "lym_copy_bandwidth", "")
+    return gbs.value
 
 
 def lib():

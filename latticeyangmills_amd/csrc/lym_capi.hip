@@ -1,4 +1,5 @@
 // C ABI of the single-lattice path (declared in include/lym.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -151,6 +152,8 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
         V *= dims[d];
     }
     if (4 * V >= (1L << 31)) return LYM_ERR_ARG;  // link ids are 32-bit Philox counters
+    const long cbytes = precision == LYM_PREC_FP64 ? 16 : 8;
+    if (9 * V * cbytes >= (1L << 31)) return LYM_ERR_ARG;  // one direction plane per buffer descriptor
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
     if (device < 0 || device >= ndev) return LYM_ERR_ARG;
@@ -465,6 +468,39 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
     }
     for (int j = 0; j < 3; j++) stage_ms[j] = st[j] / nsteps;
     stage_ms[3] = obs_at.empty() ? 0.0 : so / double(obs_at.size());
+    return LYM_OK;
+}
+
+int lym_copy_bandwidth(int device, int64_t bytes, int reps, double *gbs)
+{
+    if (bytes < 4096 || reps < 1 || !gbs) return LYM_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
+    if (hipSetDevice(device) != hipSuccess) return LYM_ERR_HIP;
+    const long n = long(bytes / 16);
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, 16 * size_t(n)) != hipSuccess) return LYM_ERR_NOMEM;
+    if (hipMalloc(&b, 16 * size_t(n)) != hipSuccess) {
+        hipFree(a);
+        return LYM_ERR_NOMEM;
+    }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    const unsigned nb = unsigned(std::min<long>((n + 255) / 256, 1L << 20));
+    lym::k_stream_copy<<<nb, 256>>>(a, b, n);  // first touch
+    hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; r++) lym::k_stream_copy<<<nb, 256>>>(a, b, n);
+    hipEventRecord(e1, nullptr);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return LYM_ERR_HIP;
+    *gbs = 2.0 * 16.0 * double(n) * reps / (ms * 1e-3) / 1e9;
     return LYM_OK;
 }
 

@@ -45,6 +45,9 @@ template <typename R> struct ah8 {
 
 #define LYM_DI __device__ __forceinline__
 
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename R> LYM_DI cplx<R> cmk(R a, R b) { return cplx<R>{a, b}; }
 template <typename R> LYM_DI cplx<R> conj(cplx<R> a) { return cplx<R>{a.re, -a.im}; }
 
@@ -201,27 +204,83 @@ template <typename To, typename From> LYM_DI su3<To> convert(const su3<From> &a)
 
 // ------------------------------------------------------------ HBM access ---
 
-template <typename R, typename S>
-LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, long s)
+// Link and accumulator accesses go through buffer resources: one descriptor
+// per direction plane (scalar registers), the matrix-element offset k*V*size
+// as the scalar soffset and the site's byte offset as the one vector offset
+// shared by all elements.  No per-element address arithmetic runs in the
+// vector ALU, and an access outside the plane reads 0 instead of faulting.
+// Requires 9 * V * sizeof(complex) < 2^31 (checked when a lattice is created).
+constexpr int kBufferDword3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
+
+template <typename T> LYM_DI __amdgpu_buffer_rsrc_t make_rsrc(const T *base, long bytes)
 {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, int(bytes), kBufferDword3);
+}
+
+template <typename T> LYM_DI T buf_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <> LYM_DI float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(voff), int(soff), 0));
+}
+template <> LYM_DI double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0));
+}
+template <> LYM_DI float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, int(voff), int(soff), 0));
+}
+template <> LYM_DI double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, int(voff), int(soff), 0));
+}
+
+template <typename T> LYM_DI void buf_store(T v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <> LYM_DI void buf_store<float>(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, int(voff), int(soff), 0);
+}
+template <> LYM_DI void buf_store<double>(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(voff), int(soff), 0);
+}
+template <> LYM_DI void buf_store<float2>(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, int(voff), int(soff), 0);
+}
+template <> LYM_DI void buf_store<double2>(double2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, int(voff), int(soff), 0);
+}
+
+// One link of a direction plane: 9 complexes at plane[k*V + s], k = 0..8.
+template <typename R, typename S>
+LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, unsigned s)
+{
+    using F2 = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 9 * V * long(sizeof(F2)));
+    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
     su3<R> m;
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        vec2_t<S> v = plane[k * V + s];
+        const F2 v = buf_load<F2>(r, off, unsigned(k) * es);
         m.e[k] = cplx<R>{R(v.x), R(v.y)};
     }
     return m;
 }
 
 template <typename S, typename R>
-LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, long s, const su3<R> &m)
+LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, unsigned s, const su3<R> &m)
 {
+    using F2 = vec2_t<S>;
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 9 * V * long(sizeof(F2)));
+    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        vec2_t<S> v;
+        F2 v;
         v.x = S(m.e[k].re);
         v.y = S(m.e[k].im);
-        plane[k * V + s] = v;
+        buf_store<F2>(v, r, off, unsigned(k) * es);
     }
 }
 
@@ -287,18 +346,75 @@ LYM_DI su3<double> herm_square(const herm8<double> &q)
     return r;
 }
 
+// E = f0 + f1 Q + f2 Q^2 for Hermitian traceless Q (d2 = -d0 - d1), Q^2 from herm_square
+LYM_DI su3<double> exp_assemble(const herm8<double> &q, const su3<double> &q2, double d2, cplx<double> f0,
+                                cplx<double> f1, cplx<double> f2)
+{
+    su3<double> r;
+    r.e[0] = cplx<double>{f0.re + f1.re * q.d0 + f2.re * q2.e[0].re, f0.im + f1.im * q.d0 + f2.im * q2.e[0].re};
+    r.e[4] = cplx<double>{f0.re + f1.re * q.d1 + f2.re * q2.e[4].re, f0.im + f1.im * q.d1 + f2.im * q2.e[4].re};
+    r.e[8] = cplx<double>{f0.re + f1.re * d2 + f2.re * q2.e[8].re, f0.im + f1.im * d2 + f2.im * q2.e[8].re};
+    r.e[1] = cfma(f2, q2.e[1], cfma(f1, q.o01, cplx<double>{0.0, 0.0}));
+    r.e[2] = cfma(f2, q2.e[2], cfma(f1, q.o02, cplx<double>{0.0, 0.0}));
+    r.e[5] = cfma(f2, q2.e[5], cfma(f1, q.o12, cplx<double>{0.0, 0.0}));
+    r.e[3] = cfma(f2, q2.e[3], cfma(f1, conj(q.o01), cplx<double>{0.0, 0.0}));
+    r.e[6] = cfma(f2, q2.e[6], cfma(f1, conj(q.o02), cplx<double>{0.0, 0.0}));
+    r.e[7] = cfma(f2, q2.e[7], cfma(f1, conj(q.o12), cplx<double>{0.0, 0.0}));
+    return r;
+}
+
+// exp(iQ) by its power series reduced with Cayley-Hamilton: for traceless Q,
+// Q^3 = c1 Q + c0 1 (c1 = Tr Q^2 / 2, c0 = det Q), so Q^n = a_n + b_n Q + d_n Q^2
+// with a_{n+1} = c0 d_n, b_{n+1} = a_n + c1 d_n, d_{n+1} = b_n, and
+// f_j = sum_n i^n coef_{j,n} / n!.  The spectral radius is at most
+// sqrt(4 c1 / 3); for c1 <= 0.3 the 18 terms kept leave a truncation error below
+// 1e-19, so this agrees with the Morningstar-Peardon form to rounding and
+// needs no transcendental function.
+constexpr double kSeriesC1Max = 0.3;
+constexpr int kSeriesTerms = 18;
+
+LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, double d2, double c1)
+{
+    cplx<double> p = cfma(q.o01, q.o12, cplx<double>{0.0, 0.0});
+    const double re3 = p.re * q.o02.re + p.im * q.o02.im;
+    const double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
+    const double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
+    const double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
+    const double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
+    double a = 1.0, b = 0.0, d = 0.0, inv_fact = 1.0;
+    double f0r = 0, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
+#pragma unroll
+    for (int n = 0; n <= kSeriesTerms; n++) {
+        if (n > 0) inv_fact *= 1.0 / double(n);
+        const double ta = a * inv_fact, tb = b * inv_fact, td = d * inv_fact;
+        switch (n & 3) {  // i^n = 1, i, -1, -i
+        case 0: f0r += ta; f1r += tb; f2r += td; break;
+        case 1: f0i += ta; f1i += tb; f2i += td; break;
+        case 2: f0r -= ta; f1r -= tb; f2r -= td; break;
+        default: f0i -= ta; f1i -= tb; f2i -= td; break;
+        }
+        const double an = c0 * d, bn = fma(c1, d, a);
+        d = b;
+        a = an;
+        b = bn;
+    }
+    return exp_assemble(q, q2, d2, cplx<double>{f0r, f0i}, cplx<double>{f1r, f1i}, cplx<double>{f2r, f2i});
+}
+
 // exp(iQ) for Hermitian traceless Q: Cayley-Hamilton form of Morningstar &
 // Peardon (hep-lat/0311018), the algorithm of the reference's exp(SU3)
 // (src/Math/su3.cpp:96-183): c0 = det Q = Tr Q^3 / 3, c1 = Tr Q^2 / 2,
 // theta = acos(c0/c0max), u, w, xi0(w) (series for |w| < 0.05), h0..h2,
 // the c0 < 0 symmetry, f_j = h_j / (9u^2 - w^2), exp(iQ) = f0 + f1 Q + f2 Q^2.
 // Q == 0 gives the identity (the reference divides 0/0 there), and c0/c0max
-// is clamped to 1 against rounding.
+// is clamped to 1 against rounding.  Small Q (c1 <= kSeriesC1Max) takes the
+// equivalent power series above.
 LYM_DI su3<double> exp_iq(const herm8<double> &q)
 {
     const double d2 = -q.d0 - q.d1;
     su3<double> q2 = herm_square(q);
     const double c1raw = 0.5 * (q2.e[0].re + q2.e[4].re + q2.e[8].re);
+    if (c1raw <= kSeriesC1Max) return exp_iq_series(q, q2, d2, c1raw);
     const bool tiny = !(c1raw > 1e-40);  // |Q| < 1e-20: exp(iQ) = 1 + iQ to double precision
     const double c1 = tiny ? 1.0 : c1raw;
     // det Q = d0 d1 d2 + 2 Re(o01 o12 conj(o02)) - d0|o12|^2 - d1|o02|^2 - d2|o01|^2
@@ -347,18 +463,7 @@ LYM_DI su3<double> exp_iq(const herm8<double> &q)
         f1 = cplx<double>{0.0, 1.0};
         f2 = cplx<double>{0.0, 0.0};
     }
-    // E = f0 + f1 Q + f2 Q^2 with Q, Q^2 Hermitian
-    su3<double> r;
-    r.e[0] = cplx<double>{f0.re + f1.re * q.d0 + f2.re * q2.e[0].re, f0.im + f1.im * q.d0 + f2.im * q2.e[0].re};
-    r.e[4] = cplx<double>{f0.re + f1.re * q.d1 + f2.re * q2.e[4].re, f0.im + f1.im * q.d1 + f2.im * q2.e[4].re};
-    r.e[8] = cplx<double>{f0.re + f1.re * d2 + f2.re * q2.e[8].re, f0.im + f1.im * d2 + f2.im * q2.e[8].re};
-    r.e[1] = cfma(f2, q2.e[1], cfma(f1, q.o01, cplx<double>{0.0, 0.0}));
-    r.e[2] = cfma(f2, q2.e[2], cfma(f1, q.o02, cplx<double>{0.0, 0.0}));
-    r.e[5] = cfma(f2, q2.e[5], cfma(f1, q.o12, cplx<double>{0.0, 0.0}));
-    r.e[3] = cfma(f2, q2.e[3], cfma(f1, conj(q.o01), cplx<double>{0.0, 0.0}));
-    r.e[6] = cfma(f2, q2.e[6], cfma(f1, conj(q.o02), cplx<double>{0.0, 0.0}));
-    r.e[7] = cfma(f2, q2.e[7], cfma(f1, conj(q.o12), cplx<double>{0.0, 0.0}));
-    return r;
+    return exp_assemble(q, q2, d2, f0, f1, f2);
 }
 
 // Z = (i/2) [X - X^+]_traceless for X = U * Omega
@@ -401,27 +506,30 @@ LYM_DI herm8<double> h8_scale(double a, const herm8<double> &x)
     return r;
 }
 
-// RK accumulator in HBM: acc[(mu*8 + k)*V + s], k = d0, d1, o01.re, o01.im, o02.re, o02.im, o12.re, o12.im
-template <typename S> LYM_DI herm8<double> load_h8(const S *__restrict__ a, long V, long s)
+// RK accumulator in HBM: acc[(mu*8 + k)*V + s], k = d0, d1, o01.re, o01.im,
+// o02.re, o02.im, o12.re, o12.im (buffer addressing as load_link)
+template <typename S> LYM_DI herm8<double> load_h8(const S *__restrict__ a, long V, unsigned s)
 {
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
+    const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = double(buf_load<S>(r, off, unsigned(k) * es));
     herm8<double> z;
-    z.d0 = double(a[0 * V + s]);
-    z.d1 = double(a[1 * V + s]);
-    z.o01 = cplx<double>{double(a[2 * V + s]), double(a[3 * V + s])};
-    z.o02 = cplx<double>{double(a[4 * V + s]), double(a[5 * V + s])};
-    z.o12 = cplx<double>{double(a[6 * V + s]), double(a[7 * V + s])};
+    z.d0 = v[0];
+    z.d1 = v[1];
+    z.o01 = cplx<double>{v[2], v[3]};
+    z.o02 = cplx<double>{v[4], v[5]};
+    z.o12 = cplx<double>{v[6], v[7]};
     return z;
 }
-template <typename S> LYM_DI void store_h8(S *__restrict__ a, long V, long s, const herm8<double> &z)
+template <typename S> LYM_DI void store_h8(S *__restrict__ a, long V, unsigned s, const herm8<double> &z)
 {
-    a[0 * V + s] = S(z.d0);
-    a[1 * V + s] = S(z.d1);
-    a[2 * V + s] = S(z.o01.re);
-    a[3 * V + s] = S(z.o01.im);
-    a[4 * V + s] = S(z.o02.re);
-    a[5 * V + s] = S(z.o02.im);
-    a[6 * V + s] = S(z.o12.re);
-    a[7 * V + s] = S(z.o12.im);
+    const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
+    const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
+    const double v[8] = {z.d0, z.d1, z.o01.re, z.o01.im, z.o02.re, z.o02.im, z.o12.re, z.o12.im};
+#pragma unroll
+    for (int k = 0; k < 8; k++) buf_store<S>(S(v[k]), r, off, unsigned(k) * es);
 }
 
 // Project a near-unitary matrix back onto SU(3): Gram-Schmidt on rows 0, 1,

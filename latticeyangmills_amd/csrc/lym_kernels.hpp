@@ -262,6 +262,12 @@ static __global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__re
 
 // ------------------------------------------------ layout and start fields ---
 
+// streaming copy b = a (grid-stride), the HBM calibration kernel
+static __global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = a[i];
+}
+
 // clears counters with a kernel (a memset node misbehaves on graph replays)
 static __global__ void k_zero_u64(unsigned long long *p, int n)
 {

@@ -1,0 +1,128 @@
+// Command-line front-end over the GPU App classes, in place of the reference's
+// JSON-driven main (src/MainPrograms/LatticeQCDMain.cpp:5-64) for its FLOW and
+// PGFG apps:
+//   lym_app flow --lattice Lx Ly Lz Lt --eps E --tau T --input DIR --output DIR
+//                [--precision fp64|mixed] [--measure-every K]
+//   lym_app generate --lattice Lx Ly Lz Lt --beta B --eps E --therm N --steps M
+//                --confs K [--start Cold|Hot] [--seed S] [--nhit H] --output DIR
+//   lym_app io-selftest DIR   (host only: writes and reads back a .bin file)
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "Actions/gpupuregauge.h"
+#include "Apps/gpugaugefieldfactory.h"
+#include "Apps/gpuwilsonflow.h"
+#include "InputOutput/confio.h"
+#include "Observables/gpusuperobs.h"
+
+namespace {
+struct Args {
+    std::array<int, 4> lat{{0, 0, 0, 0}};
+    double eps = 0.01, tau = 1.0, beta = 6.0;
+    int therm = 0, steps = 0, confs = 1, nhit = 10, every = 1, precision = LYM_PREC_FP64;
+    unsigned long long seed = 0;
+    std::string input = ".", output = ".", start = "Cold";
+};
+
+int usage()
+{
+    fprintf(stderr, "usage: lym_app flow|generate|io-selftest ... (see lym_app.cpp)\n");
+    return 2;
+}
+
+bool parse(int argc, char **argv, Args &a)
+{
+    for (int i = 2; i < argc; i++) {
+        std::string k = argv[i];
+        auto next = [&](void) -> const char * { return (i + 1 < argc) ? argv[++i] : nullptr; };
+        const char *v = nullptr;
+        if (k == "--lattice") {
+            for (int d = 0; d < 4; d++) {
+                if (!(v = next())) return false;
+                a.lat[d] = atoi(v);
+            }
+            continue;
+        }
+        if (!(v = next())) return false;
+        if (k == "--eps") a.eps = atof(v);
+        else if (k == "--tau") a.tau = atof(v);
+        else if (k == "--beta") a.beta = atof(v);
+        else if (k == "--therm") a.therm = atoi(v);
+        else if (k == "--steps") a.steps = atoi(v);
+        else if (k == "--confs") a.confs = atoi(v);
+        else if (k == "--nhit") a.nhit = atoi(v);
+        else if (k == "--measure-every") a.every = atoi(v);
+        else if (k == "--seed") a.seed = strtoull(v, nullptr, 10);
+        else if (k == "--input") a.input = v;
+        else if (k == "--output") a.output = v;
+        else if (k == "--start") a.start = v;
+        else if (k == "--precision") a.precision = (std::string(v) == "mixed") ? LYM_PREC_MIXED : LYM_PREC_FP64;
+        else return false;
+    }
+    return true;
+}
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) return usage();
+    const std::string cmd = argv[1];
+    try {
+        if (cmd == "io-selftest") {
+            if (argc < 3) return usage();
+            const std::array<int, 4> dims{{2, 4, 2, 4}};
+            GluonField a(dims), b(dims);
+            std::mt19937_64 rng(1);
+            std::uniform_real_distribution<double> u(-1, 1);
+            for (int mu = 0; mu < 4; mu++)
+                for (auto &m : a[mu].lattice)
+                    for (auto &x : m.mat) x = u(rng);
+            const std::string f = std::string(argv[2]) + "/selftest.bin";
+            LatticeIO::OutputConf::writeConf(a, f.c_str());
+            LatticeIO::InputConf::readConf(b, f.c_str());
+            for (int mu = 0; mu < 4; mu++)
+                for (int s = 0; s < a[mu].sites; s++)
+                    if (a[mu].at(s).mat != b[mu].at(s).mat) {
+                        fprintf(stderr, "mismatch mu=%d s=%d\n", mu, s);
+                        return 1;
+                    }
+            printf("io-selftest ok %s\n", f.c_str());
+            return 0;
+        }
+        Args a;
+        if (!parse(argc, argv, a) || a.lat[0] < 2) return usage();
+        if (cmd == "flow") {
+            GpuWilsonFlow app(a.tau, a.eps, a.precision, a.every);
+            GpuSuperObs obs(a.precision);
+            GpuPureGauge act(a.beta, a.precision);
+            app.setAction(&act);
+            app.addObservable(&obs);
+            app.createLattice(a.lat);
+            app.setInputDir(a.input);
+            app.setOutputDir(a.output);
+            app.execute();
+            printf("flowed %zu configuration(s) to t=%g\n", app.inputList().size(), a.tau);
+            return 0;
+        }
+        if (cmd == "generate") {
+            GpuGaugeFieldFactory app(a.steps, a.therm, a.confs, a.eps, a.start, a.beta, a.seed, a.nhit);
+            GpuSuperObs obs(LYM_PREC_FP64);
+            GpuPureGauge act(a.beta);
+            app.setAction(&act);
+            app.addObservable(&obs);
+            app.createLattice(a.lat);
+            app.setOutputDir(a.output);
+            app.execute();
+            printf("generated %d configuration(s), acceptance %.4f, last plaquette %.12f\n", a.confs,
+                   app.acceptance(), obs.plaq);
+            return 0;
+        }
+    } catch (const std::exception &e) {
+        fprintf(stderr, "lym_app: %s\n", e.what());
+        return 1;
+    }
+    return usage();
+}

@@ -1,0 +1,59 @@
+"""The C++ host classes (GpuWilsonFlow, GpuGaugeFieldFactory, GpuSuperObs,
+GpuPureGauge) through the lym_app front-end, and the reference's .bin
+configuration format (src/InputOutput/inputconf.cpp:114-155)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APP = os.path.join(ROOT, "latticeyangmills_amd", "lym_app")
+
+
+def read_bin(path, dims):
+    """.bin file: per site (x fastest) the four links, 18 doubles each -> (4, V, 18)"""
+    V = int(np.prod(dims))
+    raw = np.fromfile(path, dtype=np.float64)
+    assert raw.size == V * 4 * 18
+    return np.ascontiguousarray(raw.reshape(V, 4, 18).transpose(1, 0, 2))
+
+
+def run(*args):
+    r = subprocess.run([APP, *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_io_selftest(tmp_path):
+    assert "io-selftest ok" in run("io-selftest", tmp_path)
+    a = read_bin(tmp_path / "selftest.bin", (2, 4, 2, 4))
+    assert a.shape == (4, 64, 18) and np.all(np.abs(a) <= 1)
+
+
+@pytest.mark.gpu
+def test_generate_then_flow(tmp_path):
+    from latticeyangmills_amd import LatticeContext
+    dims = (4, 4, 4, 8)
+    confs, out = tmp_path / "confs", tmp_path / "out"
+    run("generate", "--lattice", *dims, "--beta", 6.0, "--eps", 0.24, "--therm", 5, "--steps", 4, "--confs", 2,
+        "--start", "Hot", "--seed", 3, "--output", confs)
+    files = sorted(os.listdir(confs))
+    assert files == ["conf0000.bin", "conf0001.bin"]
+    # conf0000 = hot start + 5 thermalisation + 2 sampling sweeps (NCorr = steps/confs)
+    with LatticeContext(dims, "fp64") as ctx:
+        ctx.set_hot(3)
+        ctx.metropolis(6.0, 0.24, 10, 7, 0, "exph", 3)
+        assert np.array_equal(ctx.download(), read_bin(confs / "conf0000.bin", dims))
+    run("flow", "--lattice", *dims, "--eps", 0.02, "--tau", 0.2, "--input", confs, "--output", out)
+    for k, f in enumerate(files):
+        got = np.loadtxt(out / "flow" / f"conf{k:04d}.dat")
+        want, _ = orc.flow(read_bin(confs / f, dims), dims, 0.02, 0.2, 1, 0)
+        scale = orc.observables(read_bin(confs / f, dims), dims)[3]
+        assert got.shape == want.shape == (10, 4)
+        assert np.allclose(got[:, 0], want[:, 0], rtol=0, atol=1e-15)
+        assert np.all(np.abs(got[:, 1] / want[:, 1] - 1) < 1e-12)
+        assert np.all(np.abs(got[:, 3] / want[:, 3] - 1) < 1e-12)
+        assert np.all(np.abs(got[:, 2] - want[:, 2]) < 1e-10 * scale)

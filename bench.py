@@ -106,6 +106,54 @@ def cpu_baseline(beta, seed, budget_s=15.0):
             "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
 
 
+BATCH_DIMS = [(8, 8, 8, 8)] * 16 + [(12, 12, 12, 12)] * 16 + [(16, 16, 16, 16)] * 16 + [(16, 16, 16, 32)] * 16
+
+
+def batched_ensemble(tune_cache, every=10):
+    """64 uneven lattices (16 each of 8^4, 12^4, 16^4, 16^3x32), beta = 6.0,
+    Haar hot starts with seeds 0..63, 100 Metropolis sweeps, then the mixed
+    flow eps = 0.01 to t = 2.0 (199 steps, observables every `every` steps) as
+    one persistent launch per RK stage, graph-captured, autotuned; against the
+    64 lattices flowed one after another by the plain FP64 single-lattice path."""
+    from latticeyangmills_amd import LatticeContext
+    from latticeyangmills_amd.batch import BatchContext
+    eps, tau = 0.01, 2.0
+    seeds = list(range(len(BATCH_DIMS)))
+    with BatchContext(BATCH_DIMS, "fp64") as th:
+        th.set_hot(seeds)
+        th.metropolis(6.0, 0.24, 10, 100, seeds)
+        confs = [th.download(c) for c in range(len(BATCH_DIMS))]
+    with BatchContext(BATCH_DIMS, "mixed") as b:
+        for c, L in enumerate(confs):
+            b.upload(c, L)
+        tuner = b.autotune(tune_cache)
+        b.flow(eps, tau, every, use_graph=True)  # captures the graph
+        for c, L in enumerate(confs):
+            b.upload(c, L)
+        t0 = time.perf_counter()
+        series_b = b.flow(eps, tau, every, use_graph=True)
+        t_batch = time.perf_counter() - t0
+    ctxs = []
+    for c, L in enumerate(confs):
+        ctx = LatticeContext(BATCH_DIMS[c], "fp64")
+        ctx.upload(L)
+        ctxs.append(ctx)
+    ctxs[0].flow(eps, 0.05, every)  # warm the code objects
+    ctxs[0].upload(confs[0])
+    t0 = time.perf_counter()
+    series_s = [ctx.flow(eps, tau, every) for ctx in ctxs]
+    t_seq = time.perf_counter() - t0
+    for ctx in ctxs:
+        ctx.close()
+    e_err = max(float(np.nanmax(np.abs(series_b[c][:, 3] / series_s[c][:, 3] - 1))) for c in range(len(confs)))
+    n = len(BATCH_DIMS)
+    return {"configs_per_s": n / t_batch, "seconds": t_batch, "sequential_plain_fp64_configs_per_s": n / t_seq,
+            "sequential_seconds": t_seq, "speedup": t_seq / t_batch, "tuner": tuner,
+            "max_rel_err_E_vs_fp64": e_err,
+            "workload": "64 lattices: 16 x 8^4, 16 x 12^4, 16 x 16^4, 16 x 16^3x32; beta 6.0; Haar seeds 0..63; "
+                        f"100 sweeps; mixed flow eps {eps} to t {tau} (199 steps), observables every {every}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +164,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--sweeps", type=int, default=None, help="override the Metropolis sweep count")
     ap.add_argument("--calibrate", action="store_true", help="also time a 1 GiB streaming copy (HBM probe)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
@@ -144,6 +194,7 @@ def main():
     for prec in ([w["precision"], "fp64"] if w["precision"] != "fp64" else ["fp64"]):
         with LatticeContext(dims, prec) as ctx:
             ctx.upload(links)
+            tuner = ctx.autotune(args.tune_cache)
             if args.warmup:
                 ctx.flow_steps(eps, args.warmup)
                 ctx.synchronize()
@@ -151,8 +202,8 @@ def main():
             series, el = timed_flow(ctx, eps, tau, every)
             ctx.upload(links)
             total_ms, st = ctx.profile_flow(eps, args.profile_steps, every)
-        results[prec] = dict(series=series, seconds=el, stage_ms=st, ms_per_step=1e3 * el / steps)
-        log(f"[bench] {prec}: {steps} steps in {el:.3f} s ({1e3 * el / steps:.3f} ms/step), stage ms {st}")
+        results[prec] = dict(series=series, seconds=el, stage_ms=st, ms_per_step=1e3 * el / steps, tuner=tuner)
+        log(f"[bench] {prec}: {steps} steps in {el:.3f} s ({1e3 * el / steps:.3f} ms/step), stage ms {st}, {tuner}")
 
     main_prec = w["precision"]
     r = results[main_prec]
@@ -174,7 +225,8 @@ def main():
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_flow_stage (RK stages 1+2+3 per step)",
                      "stage_ms": [float(x) for x in st[:3]], "observables_ms": float(st[3]),
-                     "valu_tflops": 4 * V * FLOP_PER_LINK_UPDATE / (step_kernel_ms * 1e-3) / 1e12},
+                     "valu_tflops": 4 * V * FLOP_PER_LINK_UPDATE / (step_kernel_ms * 1e-3) / 1e12,
+                     "schedule": r["tuner"]},
     }
     if "fp64" in results and main_prec != "fp64":
         f = results["fp64"]
@@ -186,13 +238,16 @@ def main():
         t0m, t064 = t0_from_series(sm, V), t0_from_series(s64, V)
         last = rows[-1]
         out["fp64"] = {"value": lups64, "ms_per_step": f["ms_per_step"], "speedup_mixed": lups / lups64,
-                       "stage_ms": [float(x) for x in f["stage_ms"][:3]]}
+                       "stage_ms": [float(x) for x in f["stage_ms"][:3]], "schedule": f["tuner"]}
         out["accuracy_vs_fp64"] = {
             "max_rel_err_E": e_err, "max_abs_err_Q": q_err,
             "t0_a2_mixed": t0m, "t0_a2_fp64": t064,
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),
             "t2E_at_tmax": s64[last, 0] ** 2 * e_mean(s64[last, 3], V), "Q_at_tmax": s64[last, 2],
         }
+    if not args.no_batch:
+        out["batched"] = batched_ensemble(args.tune_cache)
+        log(f"[bench] batched: {out['batched']}")
     if args.calibrate:
         from latticeyangmills_amd.capi import copy_bandwidth
         out["hbm_copy_gbs"] = copy_bandwidth(1 << 30, 10)

@@ -109,6 +109,23 @@ int lym_synchronize(lym_ctx *ctx);
 int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, double *total_ms,
                      double stage_ms[4]);
 
+/* Work schedule of the stencil kernels (no counterpart in the reference,
+ * whose loops run in site order on one core).  kind 0 = LYM_SCHED_LINEAR
+ * (workgroup b takes 64-site run b), 1 = LYM_SCHED_CONTIG (each of the 8 XCDs
+ * takes a contiguous eighth), 2 = LYM_SCHED_COLUMN (each XCD marches columns of
+ * about `target` sites of the (x, y, z) volume along t, so neighbouring time
+ * slices stay in its L2).  Results do not depend on the schedule.  Default:
+ * COLUMN with 2048 (FP64) / 4096 (mixed) sites. */
+#define LYM_SCHED_LINEAR 0
+#define LYM_SCHED_CONTIG 1
+#define LYM_SCHED_COLUMN 2
+int lym_set_schedule(lym_ctx *ctx, int kind, int target);
+/* Times every candidate schedule on the context's own field (RK stage 1,
+ * which leaves the field unchanged) and keeps the fastest; cache_path (may be
+ * NULL) holds one line per lattice shape and precision; summary receives the
+ * choice. */
+int lym_autotune(lym_ctx *ctx, const char *cache_path, char *summary, int summary_len);
+
 /* Measurement helper with no counterpart in the reference: a streaming
  * device-to-device copy of `bytes` bytes repeated `reps` times on GPU
  * `device`; *gbs = (read + written bytes) / mean kernel time.  Used to confirm

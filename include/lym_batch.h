@@ -12,21 +12,20 @@
 #define LYM_BATCH_H
 #include <stdint.h>
 
+#include "lym.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
 typedef struct lym_batch lym_batch;
 
-#define LYM_SCHED_LINEAR 0 /* workgroup k takes tiles k, k+G, ... */
-#define LYM_SCHED_XCD 1    /* tiles renumbered so each XCD walks a contiguous run */
-
 /* nconf lattices, dims = nconf x 4 extents; all start cold.  The lattices'
  * list replaces WilsonFlow's m_inputConfList (src/Apps/wilsonflow.cpp:96-109). */
 int lym_batch_create(lym_batch **b, int nconf, const int *dims, int precision, int device);
 int lym_batch_destroy(lym_batch *b);
 const char *lym_batch_last_error(const lym_batch *b);
-int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *sched);
+int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *kind, int *target);
 
 /* Per-lattice transfer in the reference's GluonField layout
  * (InputConf::readConf, src/InputOutput/inputconf.cpp:114-155). */
@@ -47,9 +46,11 @@ int lym_batch_observables(lym_batch *b, double *out);
  * flow loop (all steps and measurements) as one HIP graph and replays it. */
 int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
                    int max_rows, int *rows_out);
-/* Launch shape of the stage kernel: grid = 0 means one workgroup per 64-site
- * tile, otherwise a persistent grid of that many workgroups. */
-int lym_batch_set_schedule(lym_batch *b, int grid, int sched);
+/* Launch shape of the stage kernel: the schedule kind and target as
+ * lym_set_schedule (lattices stay whole, in order); grid = 0 means one
+ * workgroup per schedule slot, otherwise a persistent grid of that many
+ * workgroups (rounded up to a multiple of 8). */
+int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target);
 /* Times the candidate shapes on the batch itself and keeps the fastest;
  * cache_path (may be NULL) holds one line per batch shape and precision.
  * summary receives a one-line description of the choice. */

@@ -3,11 +3,10 @@ import ctypes
 
 import numpy as np
 
-from .capi import CANDIDATES, LYM_OK, PRECISIONS, LymError, _D, _dptr, _links_ptrs, lib
+from .capi import CANDIDATES, LYM_OK, PRECISIONS, SCHEDULES, LymError, _D, _dptr, _links_ptrs, lib
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
-SCHEDULES = {"linear": 0, "xcd": 1}
 _sigs_done = False
 
 
@@ -20,7 +19,7 @@ def _sigs(L):
     L.lym_batch_destroy.argtypes = [_P]
     L.lym_batch_last_error.argtypes = [_P]
     L.lym_batch_last_error.restype = c.c_char_p
-    L.lym_batch_info.argtypes = [_P, c.POINTER(_I), c.POINTER(c.c_int64), c.POINTER(_I), c.POINTER(_I)]
+    L.lym_batch_info.argtypes = [_P, c.POINTER(_I), c.POINTER(c.c_int64), c.POINTER(_I), c.POINTER(_I), c.POINTER(_I)]
     L.lym_batch_upload_links.argtypes = [_P, _I, c.POINTER(_D)]
     L.lym_batch_download_links.argtypes = [_P, _I, c.POINTER(_D)]
     L.lym_batch_set_hot.argtypes = [_P, c.POINTER(c.c_uint64)]
@@ -28,7 +27,7 @@ def _sigs(L):
                                        c.POINTER(c.c_int64)]
     L.lym_batch_observables.argtypes = [_P, _D]
     L.lym_batch_flow.argtypes = [_P, c.c_double, c.c_double, _I, _I, _D, _I, c.POINTER(_I)]
-    L.lym_batch_set_schedule.argtypes = [_P, _I, _I]
+    L.lym_batch_set_schedule.argtypes = [_P, _I, _I, _I]
     L.lym_batch_autotune.argtypes = [_P, c.c_char_p, c.c_char_p, _I]
     _sigs_done = True
     return L
@@ -101,8 +100,9 @@ class BatchContext:
                                              series.shape[1], ctypes.byref(r)), "lym_batch_flow")
         return series[:, : r.value]
 
-    def set_schedule(self, grid, sched="xcd"):
-        self._check(self._lib.lym_batch_set_schedule(self._h, grid, SCHEDULES[sched]), "lym_batch_set_schedule")
+    def set_schedule(self, grid=0, kind="column", target=4096):
+        self._check(self._lib.lym_batch_set_schedule(self._h, grid, SCHEDULES[kind], target),
+                    "lym_batch_set_schedule")
 
     def autotune(self, cache_path=None):
         buf = ctypes.create_string_buffer(256)
@@ -111,8 +111,8 @@ class BatchContext:
         return buf.value.decode()
 
     def info(self):
-        n, sites, grid, sched = _I(), ctypes.c_int64(), _I(), _I()
+        n, sites, grid, kind, target = _I(), ctypes.c_int64(), _I(), _I(), _I()
         self._check(self._lib.lym_batch_info(self._h, ctypes.byref(n), ctypes.byref(sites), ctypes.byref(grid),
-                                             ctypes.byref(sched)), "lym_batch_info")
+                                             ctypes.byref(kind), ctypes.byref(target)), "lym_batch_info")
         return dict(nconf=n.value, total_sites=sites.value, grid=grid.value,
-                    sched=[k for k, v in SCHEDULES.items() if v == sched.value][0])
+                    kind=[k for k, v in SCHEDULES.items() if v == kind.value][0], target=target.value)

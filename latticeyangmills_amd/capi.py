@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "liblym.so")
 LYM_OK = 0
 ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "device out of memory", -4: "no GPU visible"}
 PRECISIONS = {"fp64": 0, "mixed": 1}
+SCHEDULES = {"linear": 0, "contig": 1, "column": 2}
 CANDIDATES = {"su2": 0, "exph": 1}
 UNIFORMS_PER_HIT = {"su2": 13, "exph": 9}
 
@@ -50,6 +51,8 @@ def _sig(lib):
     lib.lym_synchronize.argtypes = [_P]
     lib.lym_profile_flow.argtypes = [_P, c.c_double, _I, _I, _D, _D]
     lib.lym_copy_bandwidth.argtypes = [_I, c.c_int64, _I, _D]
+    lib.lym_set_schedule.argtypes = [_P, _I, _I]
+    lib.lym_autotune.argtypes = [_P, c.c_char_p, c.c_char_p, _I]
     return lib
 
 
@@ -172,6 +175,16 @@ class LatticeContext:
         self._check(self._lib.lym_profile_flow(self._h, eps, nsteps, measure_every, ctypes.byref(total),
                                                _dptr(st)), "lym_profile_flow")
         return total.value, st
+
+    # --- work schedule ----------------------------------------------------
+    def set_schedule(self, kind, target=0):
+        self._check(self._lib.lym_set_schedule(self._h, SCHEDULES[kind], target), "lym_set_schedule")
+
+    def autotune(self, cache_path=None):
+        buf = ctypes.create_string_buffer(256)
+        self._check(self._lib.lym_autotune(self._h, cache_path.encode() if cache_path else None, buf, 256),
+                    "lym_autotune")
+        return buf.value.decode()
 
     # --- Metropolis -------------------------------------------------------
     def metropolis(self, beta, eps, nhit, nsweeps, first_sweep=0, candidate="exph", seed=0, predrawn=None):

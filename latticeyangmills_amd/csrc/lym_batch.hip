@@ -5,11 +5,10 @@
 // Work list: every lattice c is cut into tiles of 64 consecutive sites; tile
 // t of the whole batch belongs to the lattice whose first tile index
 // tile0[c] <= t < tile0[c+1].  A workgroup (4 waves = 4 directions) takes a
-// tile at a time.  Two schedules map workgroups to tiles:
-//   LINEAR: tile = blockIdx.x + k * gridDim.x
-//   XCD:    consecutive workgroup ids are dispatched round-robin over the 8
-//           XCDs, so tile = (xcd * per_xcd + local) keeps runs of neighbouring
-//           tiles (which share halo links) in one XCD's L2.
+// tile at a time, following the XCD-aware schedule table of lym_sched.hpp
+// (lattices stay whole, so small lattices each live in one XCD's L2).  The
+// grid is either one workgroup per table slot or a persistent grid (a
+// multiple of 8, so a workgroup keeps its XCD) that strides over the slots.
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -25,6 +24,7 @@
 #include "lym.h"
 #include "lym_batch.h"
 #include "lym_kernels.hpp"
+#include "lym_sched.hpp"
 
 namespace lym {
 
@@ -38,7 +38,6 @@ struct ConfDesc {
 };
 
 constexpr int kMaxConf = 1024;
-enum { SCHED_LINEAR = 0, SCHED_XCD = 1 };
 
 LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
 {
@@ -53,26 +52,15 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
     return lo;
 }
 
-LYM_DI long tile_of(long k, int sched, long ntiles)
-{
-    if (sched == SCHED_LINEAR) return k;
-    // k-th tile handled by the grid, renumbered so that XCD x (= k % 8)
-    // walks the contiguous chunk [x*per, (x+1)*per)
-    const long per = (ntiles + 7) / 8;
-    const long x = k & 7, j = k >> 3;
-    return x * per + j;
-}
-
 template <typename S, int STAGE, bool REUNIT>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
-                                                          int nconf, long ntiles, int sched, double eps)
+                                                          int nconf, long ntiles, Sched sc, double eps)
 {
-    // ceil(ntiles/8)*8 slots so that every XCD chunk is walked completely
-    const long slots = (sched == SCHED_XCD) ? ((ntiles + 7) / 8) * 8 : ntiles;
+    const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
     for (long k = blockIdx.x; k < slots; k += gridDim.x) {
-        const long tile = tile_of(k, sched, ntiles);
-        if (tile >= ntiles) continue;
+        const long tile = sched_run(sc, unsigned(k));
+        if (tile < 0) continue;
         const int c = find_conf(desc, nconf, tile);
         const ConfDesc d = desc[c];
         flow_block<S, STAGE, REUNIT>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
@@ -181,7 +169,9 @@ struct lym_batch {
     double *staging = nullptr;
     long staging_sites = 0;
     unsigned long long *accepted = nullptr;
-    int grid = 0, sched = lym::SCHED_XCD;  // launch shape of the stage kernel
+    int grid = 0;  // 0: one workgroup per schedule slot, else a persistent grid
+    lym::HostSchedule sched;
+    int *d_order = nullptr;
     std::string tuned;
     hipGraphExec_t graph = nullptr;
     std::string graph_key;
@@ -213,6 +203,38 @@ int bhip(lym_batch *b, hipError_t e, const char *what)
 
 unsigned nblocks(long n, int per) { return unsigned((n + per - 1) / per); }
 
+lym::Sched dev_sched(const lym_batch *b) { return lym::Sched{b->sched.order.empty() ? nullptr : b->d_order, b->sched.per_xcd}; }
+
+unsigned stage_grid(const lym_batch *b)
+{
+    const long slots = b->sched.order.empty() ? b->ntiles : long(lym::kXcds) * b->sched.per_xcd;
+    return unsigned(b->grid > 0 ? std::min<long>(b->grid, slots) : slots);
+}
+
+void drop_graph(lym_batch *b)
+{
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_key.clear();
+}
+
+int apply_schedule(lym_batch *b, int grid, int kind, int target)
+{
+    std::vector<long> run0;
+    for (auto &d : b->desc) run0.push_back(d.tile0);
+    lym::HostSchedule h = lym::build_schedule(b->dims, run0, kind, target);
+    if (b->d_order) hipFree(b->d_order);
+    b->d_order = nullptr;
+    if (!h.order.empty()) {
+        BHIP(b, hipMalloc(&b->d_order, sizeof(int) * h.order.size()));
+        BHIP(b, hipMemcpy(b->d_order, h.order.data(), sizeof(int) * h.order.size(), hipMemcpyHostToDevice));
+    }
+    b->sched = std::move(h);
+    b->grid = grid > 0 ? ((grid + lym::kXcds - 1) / lym::kXcds) * lym::kXcds : 0;
+    drop_graph(b);
+    return LYM_OK;
+}
+
 template <typename S> int enqueue_step(lym_batch *b, double eps)
 {
     using F2 = lym::F2<S>;
@@ -220,10 +242,11 @@ template <typename S> int enqueue_step(lym_batch *b, double eps)
     F2 *y = static_cast<F2 *>(b->W[1 - b->cur]);
     S *acc = static_cast<S *>(b->acc);
     constexpr bool re = sizeof(S) == 4;
-    const unsigned g = b->grid > 0 ? unsigned(b->grid) : unsigned(b->ntiles);
-    lym::k_batch_flow_stage<S, 1, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
-    lym::k_batch_flow_stage<S, 2, re><<<g, 256, 0, b->stream>>>(y, x, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
-    lym::k_batch_flow_stage<S, 3, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, b->sched, eps);
+    const unsigned g = stage_grid(b);
+    const lym::Sched sc = dev_sched(b);
+    lym::k_batch_flow_stage<S, 1, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
+    lym::k_batch_flow_stage<S, 2, re><<<g, 256, 0, b->stream>>>(y, x, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
+    lym::k_batch_flow_stage<S, 3, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
     BLAUNCHED(b, "k_batch_flow_stage");
     b->cur = 1 - b->cur;
     return LYM_OK;
@@ -256,11 +279,7 @@ int ensure_slots(lym_batch *b, int n)
     b->obs = nullptr;
     BHIP(b, hipMalloc(&b->obs, sizeof(double) * 4 * size_t(b->nconf) * n));
     b->obs_cap = n;
-    if (b->graph) {  // the captured graph points at the old buffer
-        hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        b->graph_key.clear();
-    }
+    drop_graph(b);  // the captured graph points at the old buffer
     return LYM_OK;
 }
 
@@ -363,6 +382,8 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(ConfDesc) * nconf, hipMemcpyHostToDevice) != hipSuccess)
         return bail(LYM_ERR_HIP);
     if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
+    if (apply_schedule(b, 0, lym::SCHED_COLUMN, precision == LYM_PREC_FP64 ? 2048 : 4096) != LYM_OK)
+        return bail(LYM_ERR_NOMEM);
     // cold start
     for (int c = 0; c < nconf; c++) {
         const long V = b->desc[c].g.V;
@@ -385,7 +406,7 @@ int lym_batch_destroy(lym_batch *b)
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->graph) hipGraphExecDestroy(b->graph);
     for (void *p : {b->W[0], b->W[1], b->acc, (void *)b->partials, (void *)b->obs, (void *)b->staging,
-                    (void *)b->accepted, (void *)b->d_desc})
+                    (void *)b->accepted, (void *)b->d_desc, (void *)b->d_order})
         if (p) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -394,13 +415,14 @@ int lym_batch_destroy(lym_batch *b)
 
 const char *lym_batch_last_error(const lym_batch *b) { return b ? b->err.c_str() : "null batch"; }
 
-int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *sched)
+int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *grid, int *kind, int *target)
 {
     if (!b) return LYM_ERR_ARG;
     if (nconf) *nconf = b->nconf;
     if (total_sites) *total_sites = b->total_sites;
     if (grid) *grid = b->grid;
-    if (sched) *sched = b->sched;
+    if (kind) *kind = b->sched.kind;
+    if (target) *target = b->sched.target;
     return LYM_OK;
 }
 
@@ -524,17 +546,13 @@ int lym_batch_observables(lym_batch *b, double *out)
     return LYM_OK;
 }
 
-int lym_batch_set_schedule(lym_batch *b, int grid, int sched)
+int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target)
 {
-    if (!b || grid < 0 || (sched != lym::SCHED_LINEAR && sched != lym::SCHED_XCD)) return LYM_ERR_ARG;
-    b->grid = grid;
-    b->sched = sched;
-    if (b->graph) {
-        hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        b->graph_key.clear();
-    }
-    return LYM_OK;
+    if (!b || grid < 0 || kind < lym::SCHED_LINEAR || kind > lym::SCHED_COLUMN || (kind == lym::SCHED_COLUMN && target < 1))
+        return LYM_ERR_ARG;
+    BHIP(b, hipSetDevice(b->device));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    return apply_schedule(b, grid, kind, target);
 }
 
 int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
@@ -554,7 +572,8 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
         // the graph is recorded for fixed buffers: W[cur] -> ... ; the final
         // field lands in W[cur ^ (nsteps & 1)], so one graph per start parity
         std::ostringstream key;
-        key << eps << "|" << rows << "|" << measure_every << "|" << b->cur << "|" << b->grid << "|" << b->sched;
+        key << eps << "|" << rows << "|" << measure_every << "|" << b->cur << "|" << b->grid << "|" << b->sched.kind
+            << "|" << b->sched.target;
         if (!b->graph || b->graph_key != key.str()) {
             if (b->graph) hipGraphExecDestroy(b->graph);
             b->graph = nullptr;
@@ -606,10 +625,10 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
 
 // Times candidate launch shapes of the RK stage kernel on the batch itself
 // (stage 1 reads W[cur] and writes W[1-cur] and the accumulator, so the field
-// is left unchanged) and keeps the fastest.  Candidates: schedule LINEAR or
-// XCD x grid = one workgroup per tile, or a persistent grid of
-// {1, 2, 4, 8, 16} x 256 workgroups.  Results are cached in `cache_path`
-// (one line per batch shape) when it is non-null.
+// is left unchanged) and keeps the fastest.  Candidates: every schedule of
+// lym::schedule_candidates() with one workgroup per slot, and the best of
+// those with persistent grids of {2, 4, 8} x 256 workgroups.  Results are
+// cached in `cache_path` (one line per batch shape) when it is non-null.
 int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len)
 {
     if (!b) return LYM_ERR_ARG;
@@ -624,34 +643,29 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
             if (p != std::string::npos) cache[line.substr(0, p)] = line.substr(p + 1);
         }
     }
-    int best_grid = 0, best_sched = lym::SCHED_XCD;
+    int best_grid = 0, best_kind = lym::SCHED_COLUMN, best_target = 4096;
     double best_ms = -1;
     auto it = cache.find(key);
-    if (it != cache.end() && sscanf(it->second.c_str(), "%d %d %lf", &best_grid, &best_sched, &best_ms) == 3) {
+    if (it != cache.end() &&
+        sscanf(it->second.c_str(), "%d %d %d %lf", &best_grid, &best_kind, &best_target, &best_ms) == 4) {
         b->tuned = "cached";
     } else {
-        std::vector<std::pair<int, int>> cands;
-        for (int sched : {lym::SCHED_LINEAR, lym::SCHED_XCD}) {
-            cands.push_back({0, sched});
-            for (int m : {1, 2, 4, 8, 16})
-                if (256L * m < b->ntiles) cands.push_back({256 * m, sched});
-        }
         hipEvent_t e0, e1;
         BHIP(b, hipEventCreate(&e0));
         BHIP(b, hipEventCreate(&e1));
-        for (auto &cd : cands) {
-            const unsigned g = cd.first > 0 ? unsigned(cd.first) : unsigned(b->ntiles);
+        auto time_stage1 = [&](float *out) -> int {
             float best_c = 1e30f;
             for (int rep = 0; rep < 4; rep++) {
                 BHIP(b, hipEventRecord(e0, b->stream));
+                const unsigned g = stage_grid(b);
                 if (b->prec == LYM_PREC_FP64)
                     lym::k_batch_flow_stage<double, 1, false><<<g, 256, 0, b->stream>>>(
                         static_cast<double2 *>(b->W[b->cur]), static_cast<double2 *>(b->W[1 - b->cur]),
-                        static_cast<double *>(b->acc), b->d_desc, b->nconf, b->ntiles, cd.second, 0.0);
+                        static_cast<double *>(b->acc), b->d_desc, b->nconf, b->ntiles, dev_sched(b), 0.0);
                 else
                     lym::k_batch_flow_stage<float, 1, true><<<g, 256, 0, b->stream>>>(
                         static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
-                        static_cast<float *>(b->acc), b->d_desc, b->nconf, b->ntiles, cd.second, 0.0);
+                        static_cast<float *>(b->acc), b->d_desc, b->nconf, b->ntiles, dev_sched(b), 0.0);
                 BLAUNCHED(b, "k_batch_flow_stage");
                 BHIP(b, hipEventRecord(e1, b->stream));
                 BHIP(b, hipEventSynchronize(e1));
@@ -659,10 +673,29 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
                 BHIP(b, hipEventElapsedTime(&ms, e0, e1));
                 if (rep > 0) best_c = std::min(best_c, ms);  // first run warms caches
             }
-            if (best_ms < 0 || best_c < best_ms) {
-                best_ms = best_c;
-                best_grid = cd.first;
-                best_sched = cd.second;
+            *out = best_c;
+            return LYM_OK;
+        };
+        int rc;
+        for (auto cd : lym::schedule_candidates()) {
+            if ((rc = apply_schedule(b, 0, cd.first, cd.second))) return rc;
+            float ms;
+            if ((rc = time_stage1(&ms))) return rc;
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                best_grid = 0;
+                best_kind = cd.first;
+                best_target = cd.second;
+            }
+        }
+        const int k0 = best_kind, t0 = best_target;
+        for (int m : {2, 4, 8}) {
+            if ((rc = apply_schedule(b, 256 * m, k0, t0))) return rc;
+            float ms;
+            if ((rc = time_stage1(&ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_grid = 256 * m;
             }
         }
         hipEventDestroy(e0);
@@ -670,14 +703,14 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
         b->tuned = "measured";
         if (cache_path) {
             std::ofstream o(cache_path, std::ios::app);
-            o << key << "\t" << best_grid << " " << best_sched << " " << best_ms << "\n";
+            o << key << "\t" << best_grid << " " << best_kind << " " << best_target << " " << best_ms << "\n";
         }
     }
-    lym_batch_set_schedule(b, best_grid, best_sched);
-    if (summary && summary_len > 0) {
-        snprintf(summary, size_t(summary_len), "grid=%d sched=%s stage1_ms=%.4f source=%s tiles=%ld", best_grid,
-                 best_sched == lym::SCHED_XCD ? "xcd" : "linear", best_ms, b->tuned.c_str(), b->ntiles);
-    }
+    int rc = apply_schedule(b, best_grid, best_kind, best_target);
+    if (rc) return rc;
+    if (summary && summary_len > 0)
+        snprintf(summary, size_t(summary_len), "grid=%d schedule=%s target=%d stage1_ms=%.4f source=%s tiles=%ld",
+                 best_grid, lym::schedule_name(best_kind), best_target, best_ms, b->tuned.c_str(), b->ntiles);
     return LYM_OK;
 }
 

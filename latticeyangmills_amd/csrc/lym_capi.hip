@@ -8,6 +8,11 @@
 
 #include "lym.h"
 #include "lym_kernels.hpp"
+#include "lym_sched.hpp"
+
+#include <fstream>
+#include <map>
+#include <sstream>
 
 using lym::Geom;
 using lym::ksum;
@@ -30,6 +35,8 @@ struct lym_ctx {
     double *predrawn = nullptr;
     size_t predrawn_cap = 0;
     std::vector<hipEvent_t> events;
+    lym::HostSchedule sched;  // work schedule of the stencil kernels
+    int *d_order = nullptr;
     std::string err;
 };
 
@@ -63,21 +70,52 @@ size_t real_size(const lym_ctx *c) { return c->prec == LYM_PREC_FP64 ? sizeof(do
 
 unsigned blocks_for(long n, int per) { return unsigned((n + per - 1) / per); }
 
+lym::Sched dev_sched(const lym_ctx *c) { return lym::Sched{c->sched.order.empty() ? nullptr : c->d_order, c->sched.per_xcd}; }
+
+// workgroups of a stencil launch under the current schedule
+unsigned stencil_blocks(const lym_ctx *c)
+{
+    return c->sched.order.empty() ? blocks_for(c->g.V, lym::kSitesPerBlock) : unsigned(lym::kXcds * c->sched.per_xcd);
+}
+
+int apply_schedule(lym_ctx *c, int kind, int target)
+{
+    std::vector<std::array<int, 4>> d{{c->dims[0], c->dims[1], c->dims[2], c->dims[3]}};
+    lym::HostSchedule h = lym::build_schedule(d, {0L}, kind, target);
+    if (c->d_order) hipFree(c->d_order);
+    c->d_order = nullptr;
+    if (!h.order.empty()) {
+        LYM_HIP(c, hipMalloc(&c->d_order, sizeof(int) * h.order.size()));
+        LYM_HIP(c, hipMemcpy(c->d_order, h.order.data(), sizeof(int) * h.order.size(), hipMemcpyHostToDevice));
+    }
+    // partials: one slot per workgroup of the observables launch
+    const long nblk = h.order.empty() ? long(blocks_for(c->g.V, lym::kObsSites)) : long(lym::kXcds) * h.per_xcd;
+    if (nblk > c->nblk_obs) {
+        if (c->partials) hipFree(c->partials);
+        c->partials = nullptr;
+        LYM_HIP(c, hipMalloc(&c->partials, sizeof(ksum) * 4 * size_t(nblk)));
+    }
+    c->nblk_obs = int(nblk);
+    c->sched = std::move(h);
+    return LYM_OK;
+}
+
 template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
 {
     using F2 = lym::F2<S>;
     F2 *a = static_cast<F2 *>(c->W[c->cur]);
     F2 *b = static_cast<F2 *>(c->W[1 - c->cur]);
     S *acc = static_cast<S *>(c->acc);
-    const unsigned nb = blocks_for(c->g.V, lym::kSitesPerBlock);
+    const unsigned nb = stencil_blocks(c);
+    const lym::Sched sc = dev_sched(c);
     constexpr bool reunit = sizeof(S) == 4;
-    if (ev) hipEventRecord(ev[0], c->stream);
-    lym::k_flow_stage<S, 1, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, eps);
-    if (ev) hipEventRecord(ev[1], c->stream);
-    lym::k_flow_stage<S, 2, reunit><<<nb, 256, 0, c->stream>>>(b, a, acc, c->g, eps);
-    if (ev) hipEventRecord(ev[2], c->stream);
-    lym::k_flow_stage<S, 3, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, eps);
-    if (ev) hipEventRecord(ev[3], c->stream);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[0], c->stream));
+    lym::k_flow_stage<S, 1, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, sc, eps);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[1], c->stream));
+    lym::k_flow_stage<S, 2, reunit><<<nb, 256, 0, c->stream>>>(b, a, acc, c->g, sc, eps);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[2], c->stream));
+    lym::k_flow_stage<S, 3, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, sc, eps);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[3], c->stream));
     LYM_LAUNCHED(c, "k_flow_stage");
     c->cur = 1 - c->cur;
     return LYM_OK;
@@ -92,9 +130,9 @@ int step(lym_ctx *c, double eps, hipEvent_t *ev = nullptr)
 template <typename S> int enqueue_observables(lym_ctx *c, int slot, hipEvent_t *ev)
 {
     const lym::F2<S> *f = static_cast<const lym::F2<S> *>(c->W[c->cur]);
-    if (ev) hipEventRecord(ev[0], c->stream);
-    lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, c->partials);
-    if (ev) hipEventRecord(ev[1], c->stream);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[0], c->stream));
+    lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
+    if (ev) LYM_HIP(c, hipEventRecord(ev[1], c->stream));
     lym::k_reduce_partials<<<1, 256, 0, c->stream>>>(c->partials, c->nblk_obs, c->obs + 4 * slot);
     LYM_LAUNCHED(c, "k_observables");
     return LYM_OK;
@@ -183,6 +221,8 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
     if (hipMalloc(&c->staging, sizeof(double) * 18 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
+    if (apply_schedule(c, lym::SCHED_COLUMN, precision == LYM_PREC_FP64 ? 2048 : 4096) != LYM_OK)
+        return bail(LYM_ERR_NOMEM);
     int rc = lym_set_cold(c);
     if (rc != LYM_OK) return bail(rc);
     *out = c;
@@ -195,7 +235,7 @@ int lym_destroy(lym_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
-                    (void *)c->accepted, (void *)c->predrawn})
+                    (void *)c->accepted, (void *)c->predrawn, (void *)c->d_order})
         if (p) hipFree(p);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -468,6 +508,79 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
     }
     for (int j = 0; j < 3; j++) stage_ms[j] = st[j] / nsteps;
     stage_ms[3] = obs_at.empty() ? 0.0 : so / double(obs_at.size());
+    return LYM_OK;
+}
+
+int lym_set_schedule(lym_ctx *c, int kind, int target)
+{
+    if (!c) return LYM_ERR_ARG;
+    if (kind < lym::SCHED_LINEAR || kind > lym::SCHED_COLUMN || (kind == lym::SCHED_COLUMN && target < 1))
+        return fail(c, LYM_ERR_ARG, "bad schedule");
+    LYM_HIP(c, hipSetDevice(c->device));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    return apply_schedule(c, kind, target);
+}
+
+int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_len)
+{
+    if (!c) return LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    std::ostringstream key;
+    key << "gfx950|single|prec" << c->prec << "|" << c->dims[0] << "x" << c->dims[1] << "x" << c->dims[2] << "x"
+        << c->dims[3];
+    std::map<std::string, std::string> cache;
+    if (cache_path) {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            auto p = line.find('\t');
+            if (p != std::string::npos) cache[line.substr(0, p)] = line.substr(p + 1);
+        }
+    }
+    int best_kind = -1, best_target = 0;
+    double best_ms = -1;
+    const char *source = "cached";
+    auto it = cache.find(key.str());
+    if (it == cache.end() || sscanf(it->second.c_str(), "%d %d %lf", &best_kind, &best_target, &best_ms) != 3) {
+        source = "measured";
+        int rc = ensure_events(c, 2);
+        if (rc) return rc;
+        for (auto cand : lym::schedule_candidates()) {
+            if ((rc = apply_schedule(c, cand.first, cand.second))) return rc;
+            float best_c = 1e30f;
+            for (int rep = 0; rep < 4; rep++) {  // stage 1 leaves W[cur] unchanged
+                LYM_HIP(c, hipEventRecord(c->events[0], c->stream));
+                if (c->prec == LYM_PREC_FP64)
+                    lym::k_flow_stage<double, 1, false><<<stencil_blocks(c), 256, 0, c->stream>>>(
+                        static_cast<double2 *>(c->W[c->cur]), static_cast<double2 *>(c->W[1 - c->cur]),
+                        static_cast<double *>(c->acc), c->g, dev_sched(c), 0.0);
+                else
+                    lym::k_flow_stage<float, 1, true><<<stencil_blocks(c), 256, 0, c->stream>>>(
+                        static_cast<float2 *>(c->W[c->cur]), static_cast<float2 *>(c->W[1 - c->cur]),
+                        static_cast<float *>(c->acc), c->g, dev_sched(c), 0.0);
+                LYM_LAUNCHED(c, "k_flow_stage");
+                LYM_HIP(c, hipEventRecord(c->events[1], c->stream));
+                LYM_HIP(c, hipEventSynchronize(c->events[1]));
+                float ms = 0;
+                LYM_HIP(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
+                if (rep > 0) best_c = std::min(best_c, ms);
+            }
+            if (best_ms < 0 || best_c < best_ms) {
+                best_ms = best_c;
+                best_kind = cand.first;
+                best_target = cand.second;
+            }
+        }
+        if (cache_path) {
+            std::ofstream o(cache_path, std::ios::app);
+            o << key.str() << "\t" << best_kind << " " << best_target << " " << best_ms << "\n";
+        }
+    }
+    int rc = apply_schedule(c, best_kind, best_target);
+    if (rc) return rc;
+    if (summary && summary_len > 0)
+        snprintf(summary, size_t(summary_len), "schedule=%s target=%d stage1_ms=%.4f source=%s",
+                 lym::schedule_name(best_kind), best_target, best_ms, source);
     return LYM_OK;
 }
 

@@ -12,6 +12,19 @@ namespace lym {
 
 constexpr int kSitesPerBlock = 64;
 
+// Workgroup -> run of 64 sites (see lym_sched.hpp): workgroup b runs on XCD
+// b % 8 and takes entry b / 8 of that XCD's row of the table; no table means
+// workgroup b takes run b.  Returns -1 for padding entries.
+struct Sched {
+    const int *order;
+    int per_xcd;
+};
+LYM_DI long sched_run(const Sched sc, unsigned b)
+{
+    if (!sc.order) return long(b);
+    return sc.order[(b & 7u) * unsigned(sc.per_xcd) + (b >> 3)];
+}
+
 template <typename S> using F2 = vec2_t<S>;
 
 // ---------------------------------------------------------------- staple ---
@@ -106,9 +119,11 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
 
 template <typename S, int STAGE, bool REUNIT>
 __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
-                                                    S *__restrict__ acc, Geom g, double eps)
+                                                    S *__restrict__ acc, Geom g, Sched sc, double eps)
 {
-    flow_block<S, STAGE, REUNIT>(win, wout, acc, g, (long)blockIdx.x * kSitesPerBlock, eps);
+    const long run = sched_run(sc, blockIdx.x);
+    if (run < 0) return;
+    flow_block<S, STAGE, REUNIT>(win, wout, acc, g, run * kSitesPerBlock, eps);
 }
 
 // ----------------------------------------------------------- observables ---
@@ -225,12 +240,14 @@ template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
 constexpr int kObsSites = 64;
 
 template <typename S>
-__global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, ksum *__restrict__ partials)
+__global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
+                                                     ksum *__restrict__ partials)
 {
-    const long s = (long)blockIdx.x * kObsSites + (threadIdx.x & 63);
+    const long run = sched_run(sc, blockIdx.x);
+    const long s = run * kObsSites + (threadIdx.x & 63);
     const int pair = threadIdx.x >> 6;
     ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    if (s < g.V) {
+    if (run >= 0 && s < g.V) {
         double o[3];
         if (pair == 0)
             pair_observables<S, S, 0>(f, g, s, o);

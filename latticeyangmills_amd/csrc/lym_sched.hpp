@@ -1,0 +1,107 @@
+// Work schedules of the stencil kernels (host side).
+//
+// A lattice is cut into runs of 64 consecutive sites (x fastest); one
+// workgroup handles one run.  Workgroups are dispatched round-robin over the
+// 8 XCDs of an MI355X (workgroup b runs on XCD b % 8), and each XCD has its own
+// 4 MB L2, so which runs an XCD sees decides how often a link is fetched from
+// HBM: every link is read by ~19 neighbouring link updates.  The schedule is
+// a table order[xcd * per_xcd + j] = run handled by the j-th workgroup of
+// that XCD (-1 = none), launched as 8 * per_xcd workgroups:
+//   LINEAR  no table: workgroup b takes run b (each XCD sees every 8th run);
+//   CONTIG  the run list split into 8 contiguous parts, one per XCD;
+//   COLUMN  runs ordered by (column, t, z, y, x) where a column is a block of
+//           Zc x Yc rows of the (z, y) plane of about `target` sites, so an XCD
+//           marches a narrow column along t and the previous and next time
+//           slices of the column stay in its L2; then split as CONTIG.
+// Any schedule visits every run once, so results do not depend on it.
+#pragma once
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+namespace lym {
+
+enum SchedKind { SCHED_LINEAR = 0, SCHED_CONTIG = 1, SCHED_COLUMN = 2 };
+constexpr int kXcds = 8;
+constexpr int kRunSites = 64;
+
+struct HostSchedule {
+    int kind = SCHED_LINEAR, target = 0;
+    int per_xcd = 0;         // table row length
+    std::vector<int> order;  // kXcds * per_xcd entries (empty for LINEAR)
+};
+
+// runs of one lattice in COLUMN order (run indices relative to the lattice)
+inline std::vector<int> column_order(const std::array<int, 4> &L, int target)
+{
+    const long V = long(L[0]) * L[1] * L[2] * L[3];
+    const long nruns = (V + kRunSites - 1) / kRunSites;
+    const int rows = std::max(1, target / L[0]);  // (y, z) rows per column
+    const int yc = std::min(L[1], std::max(1, int(std::lround(std::sqrt(double(rows))))));
+    const int zc = std::min(L[2], std::max(1, rows / yc));
+    const int ny = (L[1] + yc - 1) / yc;
+    const size_t n = static_cast<size_t>(nruns);
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (long r = 0; r < nruns; r++) {
+        long s = r * kRunSites;
+        const int x = int(s % L[0]);
+        s /= L[0];
+        const int y = int(s % L[1]);
+        s /= L[1];
+        const int z = int(s % L[2]);
+        const int t = int(s / L[2]);
+        const uint64_t col = uint64_t(z / zc) * ny + uint64_t(y / yc);
+        const uint64_t k = ((((col * L[3] + t) * L[2] + z) * L[1] + y) * L[0]) + x;
+        key[size_t(r)] = {k, int(r)};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> out(n);
+    for (long r = 0; r < nruns; r++) out[size_t(r)] = key[size_t(r)].second;
+    return out;
+}
+
+// Schedule over several lattices whose runs are numbered consecutively
+// (lattice c owns runs [run0[c], run0[c] + nruns(c))).  Lattices stay whole
+// and in order, so small lattices land on one XCD each.
+inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, const std::vector<long> &run0,
+                                   int kind, int target)
+{
+    HostSchedule h;
+    h.kind = kind;
+    h.target = target;
+    if (kind == SCHED_LINEAR) return h;
+    std::vector<int> all;
+    for (size_t c = 0; c < dims.size(); c++) {
+        const long V = long(dims[c][0]) * dims[c][1] * dims[c][2] * dims[c][3];
+        const long nruns = (V + kRunSites - 1) / kRunSites;
+        if (kind == SCHED_COLUMN) {
+            for (int r : column_order(dims[c], target)) all.push_back(int(run0[c] + r));
+        } else {
+            for (long r = 0; r < nruns; r++) all.push_back(int(run0[c] + r));
+        }
+    }
+    const long n = long(all.size());
+    h.per_xcd = int((n + kXcds - 1) / kXcds);
+    h.order.assign(size_t(kXcds) * h.per_xcd, -1);
+    for (int x = 0; x < kXcds; x++) {
+        const long b = n * x / kXcds, e = n * (x + 1) / kXcds;
+        for (long i = b; i < e; i++) h.order[size_t(x) * h.per_xcd + size_t(i - b)] = all[size_t(i)];
+    }
+    return h;
+}
+
+// the candidates the autotuners time
+inline std::vector<std::pair<int, int>> schedule_candidates()
+{
+    return {{SCHED_LINEAR, 0},     {SCHED_CONTIG, 0},     {SCHED_COLUMN, 1024},
+            {SCHED_COLUMN, 2048},  {SCHED_COLUMN, 4096},  {SCHED_COLUMN, 8192}};
+}
+
+inline const char *schedule_name(int kind)
+{
+    return kind == SCHED_LINEAR ? "linear" : kind == SCHED_CONTIG ? "contig" : "column";
+}
+
+}  // namespace lym

@@ -37,8 +37,9 @@ def test_upload_download_and_observables(inputs, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("grid,sched", [(0, "linear"), (0, "xcd"), (8, "xcd"), (3, "linear")])
-def test_flow_matches_single_lattice_path(inputs, precision, grid, sched):
+@pytest.mark.parametrize("grid,kind,target", [(0, "linear", 0), (0, "contig", 0), (0, "column", 64),
+                                               (8, "column", 128), (3, "linear", 0)])
+def test_flow_matches_single_lattice_path(inputs, precision, grid, kind, target):
     eps, tau, every = 0.02, 0.2, 3
     singles = []
     for c, L in enumerate(inputs):
@@ -47,7 +48,7 @@ def test_flow_matches_single_lattice_path(inputs, precision, grid, sched):
             s = ctx.flow(eps, tau, every)
             singles.append((s, ctx.download()))
     with BatchContext(DIMS, precision) as b:
-        b.set_schedule(grid, sched)
+        b.set_schedule(grid, kind, target)
         for c, L in enumerate(inputs):
             b.upload(c, L)
         series = b.flow(eps, tau, every, use_graph=False)

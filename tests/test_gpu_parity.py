@@ -180,3 +180,28 @@ def test_mixed_precision_against_fp64(therm8):
         assert rel(got[r, 1], want[r, 1]) < 1e-6, r
         assert rel(got[r, 3], want[r, 3]) < 1e-5, r
         assert abs(got[r, 2] - want[r, 2]) < 1e-4, r
+
+
+@pytest.mark.parametrize("kind,target", [("linear", 0), ("contig", 0), ("column", 64), ("column", 300)])
+def test_schedule_does_not_change_results(therm8, kind, target):
+    dims, links = therm8
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.upload(links)
+        ref = ctx.flow(0.02, 0.2, 3)
+        ref_links = ctx.download()
+        ctx.set_schedule(kind, target)
+        ctx.upload(links)
+        got = ctx.flow(0.02, 0.2, 3)
+        assert np.array_equal(ctx.download(), ref_links)
+        m = ~np.isnan(ref[:, 1])
+        assert np.allclose(got[m], ref[m], rtol=1e-14, atol=0)
+
+
+def test_autotune_picks_a_schedule(therm8, tmp_path):
+    dims, links = therm8
+    with LatticeContext(dims, "fp64") as ctx:
+        ctx.upload(links)
+        s = ctx.autotune(str(tmp_path / "tune.txt"))
+        assert "schedule=" in s and "measured" in s
+        assert "cached" in ctx.autotune(str(tmp_path / "tune.txt"))
+        assert np.array_equal(ctx.download(), links)  # tuning leaves the field alone

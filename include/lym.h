@@ -119,6 +119,7 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
 #define LYM_SCHED_LINEAR 0
 #define LYM_SCHED_CONTIG 1
 #define LYM_SCHED_COLUMN 2
+#define LYM_SCHED_LDS 16 /* or-ed into a kind: stage each 64-site run's own links in LDS */
 int lym_set_schedule(lym_ctx *ctx, int kind, int target);
 /* Times every candidate schedule on the context's own field (RK stage 1,
  * which leaves the field unchanged) and keeps the fastest; cache_path (may be

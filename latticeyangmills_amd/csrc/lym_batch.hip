@@ -52,7 +52,7 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
     return lo;
 }
 
-template <typename S, int STAGE, bool REUNIT>
+template <typename S, int STAGE, bool REUNIT, int LDS>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
                                                           int nconf, long ntiles, Sched sc, double eps)
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restric
         if (tile < 0) continue;
         const int c = find_conf(desc, nconf, tile);
         const ConfDesc d = desc[c];
-        flow_block<S, STAGE, REUNIT>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
+        flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
                                      (tile - d.tile0) * kSitesPerBlock, eps);
     }
 }
@@ -213,7 +213,7 @@ unsigned stage_grid(const lym_batch *b)
 
 void drop_graph(lym_batch *b)
 {
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     b->graph_key.clear();
 }
@@ -223,7 +223,7 @@ int apply_schedule(lym_batch *b, int grid, int kind, int target)
     std::vector<long> run0;
     for (auto &d : b->desc) run0.push_back(d.tile0);
     lym::HostSchedule h = lym::build_schedule(b->dims, run0, kind, target);
-    if (b->d_order) hipFree(b->d_order);
+    if (b->d_order) (void)hipFree(b->d_order);
     b->d_order = nullptr;
     if (!h.order.empty()) {
         BHIP(b, hipMalloc(&b->d_order, sizeof(int) * h.order.size()));
@@ -235,6 +235,24 @@ int apply_schedule(lym_batch *b, int grid, int kind, int target)
     return LYM_OK;
 }
 
+template <typename S, int STAGE>
+void launch_stage(const lym_batch *b, const lym::F2<S> *win, lym::F2<S> *wout, S *acc, double eps)
+{
+    constexpr bool re = sizeof(S) == 4;
+    const unsigned g = stage_grid(b);
+    bool xrows = true;  // every lattice's runs hold whole x-rows
+    for (auto &d : b->dims) xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
+    if (!(b->sched.kind & lym::SCHED_LDS))
+        lym::k_batch_flow_stage<S, STAGE, re, 0><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
+                                                                          b->ntiles, dev_sched(b), eps);
+    else if (xrows)
+        lym::k_batch_flow_stage<S, STAGE, re, 2><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
+                                                                          b->ntiles, dev_sched(b), eps);
+    else
+        lym::k_batch_flow_stage<S, STAGE, re, 1><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
+                                                                          b->ntiles, dev_sched(b), eps);
+}
+
 template <typename S> int enqueue_step(lym_batch *b, double eps)
 {
     using F2 = lym::F2<S>;
@@ -242,11 +260,9 @@ template <typename S> int enqueue_step(lym_batch *b, double eps)
     F2 *y = static_cast<F2 *>(b->W[1 - b->cur]);
     S *acc = static_cast<S *>(b->acc);
     constexpr bool re = sizeof(S) == 4;
-    const unsigned g = stage_grid(b);
-    const lym::Sched sc = dev_sched(b);
-    lym::k_batch_flow_stage<S, 1, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
-    lym::k_batch_flow_stage<S, 2, re><<<g, 256, 0, b->stream>>>(y, x, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
-    lym::k_batch_flow_stage<S, 3, re><<<g, 256, 0, b->stream>>>(x, y, acc, b->d_desc, b->nconf, b->ntiles, sc, eps);
+    launch_stage<S, 1>(b, x, y, acc, eps);
+    launch_stage<S, 2>(b, y, x, acc, eps);
+    launch_stage<S, 3>(b, x, y, acc, eps);
     BLAUNCHED(b, "k_batch_flow_stage");
     b->cur = 1 - b->cur;
     return LYM_OK;
@@ -275,7 +291,7 @@ int ensure_slots(lym_batch *b, int n)
 {
     if (n <= b->obs_cap) return LYM_OK;
     BHIP(b, hipStreamSynchronize(b->stream));
-    if (b->obs) hipFree(b->obs);
+    if (b->obs) (void)hipFree(b->obs);
     b->obs = nullptr;
     BHIP(b, hipMalloc(&b->obs, sizeof(double) * 4 * size_t(b->nconf) * n));
     b->obs_cap = n;
@@ -402,13 +418,13 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
 int lym_batch_destroy(lym_batch *b)
 {
     if (!b) return LYM_OK;
-    hipSetDevice(b->device);
-    if (b->stream) hipStreamSynchronize(b->stream);
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (void *p : {b->W[0], b->W[1], b->acc, (void *)b->partials, (void *)b->obs, (void *)b->staging,
                     (void *)b->accepted, (void *)b->d_desc, (void *)b->d_order})
-        if (p) hipFree(p);
-    if (b->stream) hipStreamDestroy(b->stream);
+        if (p) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
     return LYM_OK;
 }
@@ -548,7 +564,9 @@ int lym_batch_observables(lym_batch *b, double *out)
 
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target)
 {
-    if (!b || grid < 0 || kind < lym::SCHED_LINEAR || kind > lym::SCHED_COLUMN || (kind == lym::SCHED_COLUMN && target < 1))
+    const int order = kind & ~lym::SCHED_LDS;
+    if (!b || grid < 0 || (kind & ~(lym::SCHED_LDS | 3)) || order > lym::SCHED_COLUMN ||
+        (order == lym::SCHED_COLUMN && target < 1))
         return LYM_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
     BHIP(b, hipStreamSynchronize(b->stream));
@@ -575,7 +593,7 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
         key << eps << "|" << rows << "|" << measure_every << "|" << b->cur << "|" << b->grid << "|" << b->sched.kind
             << "|" << b->sched.target;
         if (!b->graph || b->graph_key != key.str()) {
-            if (b->graph) hipGraphExecDestroy(b->graph);
+            if (b->graph) (void)hipGraphExecDestroy(b->graph);
             b->graph = nullptr;
             const int cur0 = b->cur;
             BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -584,12 +602,12 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             b->cur = cur0;
             if (rc) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return rc;
             }
             if (e != hipSuccess) return bhip(b, e, "hipStreamEndCapture");
             e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             if (e != hipSuccess) return bhip(b, e, "hipGraphInstantiate");
             b->graph_key = key.str();
         }
@@ -657,15 +675,12 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
             float best_c = 1e30f;
             for (int rep = 0; rep < 4; rep++) {
                 BHIP(b, hipEventRecord(e0, b->stream));
-                const unsigned g = stage_grid(b);
                 if (b->prec == LYM_PREC_FP64)
-                    lym::k_batch_flow_stage<double, 1, false><<<g, 256, 0, b->stream>>>(
-                        static_cast<double2 *>(b->W[b->cur]), static_cast<double2 *>(b->W[1 - b->cur]),
-                        static_cast<double *>(b->acc), b->d_desc, b->nconf, b->ntiles, dev_sched(b), 0.0);
+                    launch_stage<double, 1>(b, static_cast<double2 *>(b->W[b->cur]),
+                                            static_cast<double2 *>(b->W[1 - b->cur]), static_cast<double *>(b->acc), 0.0);
                 else
-                    lym::k_batch_flow_stage<float, 1, true><<<g, 256, 0, b->stream>>>(
-                        static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
-                        static_cast<float *>(b->acc), b->d_desc, b->nconf, b->ntiles, dev_sched(b), 0.0);
+                    launch_stage<float, 1>(b, static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
+                                           static_cast<float *>(b->acc), 0.0);
                 BLAUNCHED(b, "k_batch_flow_stage");
                 BHIP(b, hipEventRecord(e1, b->stream));
                 BHIP(b, hipEventSynchronize(e1));
@@ -698,8 +713,8 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
                 best_grid = 256 * m;
             }
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         b->tuned = "measured";
         if (cache_path) {
             std::ofstream o(cache_path, std::ios::app);

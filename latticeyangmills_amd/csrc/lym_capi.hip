@@ -82,7 +82,7 @@ int apply_schedule(lym_ctx *c, int kind, int target)
 {
     std::vector<std::array<int, 4>> d{{c->dims[0], c->dims[1], c->dims[2], c->dims[3]}};
     lym::HostSchedule h = lym::build_schedule(d, {0L}, kind, target);
-    if (c->d_order) hipFree(c->d_order);
+    if (c->d_order) (void)hipFree(c->d_order);
     c->d_order = nullptr;
     if (!h.order.empty()) {
         LYM_HIP(c, hipMalloc(&c->d_order, sizeof(int) * h.order.size()));
@@ -91,7 +91,7 @@ int apply_schedule(lym_ctx *c, int kind, int target)
     // partials: one slot per workgroup of the observables launch
     const long nblk = h.order.empty() ? long(blocks_for(c->g.V, lym::kObsSites)) : long(lym::kXcds) * h.per_xcd;
     if (nblk > c->nblk_obs) {
-        if (c->partials) hipFree(c->partials);
+        if (c->partials) (void)hipFree(c->partials);
         c->partials = nullptr;
         LYM_HIP(c, hipMalloc(&c->partials, sizeof(ksum) * 4 * size_t(nblk)));
     }
@@ -100,21 +100,32 @@ int apply_schedule(lym_ctx *c, int kind, int target)
     return LYM_OK;
 }
 
+// one RK stage over the lattice with the context's schedule and kernel variant
+template <typename S, int STAGE>
+void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *acc, double eps)
+{
+    constexpr bool reunit = sizeof(S) == 4;
+    const unsigned nb = stencil_blocks(c);
+    if (!(c->sched.kind & lym::SCHED_LDS))
+        lym::k_flow_stage<S, STAGE, reunit, 0><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+    else if (lym::kSitesPerBlock % c->dims[0] == 0)  // runs hold whole x-rows
+        lym::k_flow_stage<S, STAGE, reunit, 2><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+    else
+        lym::k_flow_stage<S, STAGE, reunit, 1><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+}
+
 template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
 {
     using F2 = lym::F2<S>;
     F2 *a = static_cast<F2 *>(c->W[c->cur]);
     F2 *b = static_cast<F2 *>(c->W[1 - c->cur]);
     S *acc = static_cast<S *>(c->acc);
-    const unsigned nb = stencil_blocks(c);
-    const lym::Sched sc = dev_sched(c);
-    constexpr bool reunit = sizeof(S) == 4;
     if (ev) LYM_HIP(c, hipEventRecord(ev[0], c->stream));
-    lym::k_flow_stage<S, 1, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, sc, eps);
+    launch_stage<S, 1>(c, a, b, acc, eps);
     if (ev) LYM_HIP(c, hipEventRecord(ev[1], c->stream));
-    lym::k_flow_stage<S, 2, reunit><<<nb, 256, 0, c->stream>>>(b, a, acc, c->g, sc, eps);
+    launch_stage<S, 2>(c, b, a, acc, eps);
     if (ev) LYM_HIP(c, hipEventRecord(ev[2], c->stream));
-    lym::k_flow_stage<S, 3, reunit><<<nb, 256, 0, c->stream>>>(a, b, acc, c->g, sc, eps);
+    launch_stage<S, 3>(c, a, b, acc, eps);
     if (ev) LYM_HIP(c, hipEventRecord(ev[3], c->stream));
     LYM_LAUNCHED(c, "k_flow_stage");
     c->cur = 1 - c->cur;
@@ -148,7 +159,7 @@ int ensure_slots(lym_ctx *c, int n)
 {
     if (n <= c->obs_cap) return LYM_OK;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->obs) hipFree(c->obs);
+    if (c->obs) (void)hipFree(c->obs);
     c->obs = nullptr;
     LYM_HIP(c, hipMalloc(&c->obs, sizeof(double) * 4 * size_t(n)));
     c->obs_cap = n;
@@ -232,13 +243,13 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
 int lym_destroy(lym_ctx *c)
 {
     if (!c) return LYM_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
                     (void *)c->accepted, (void *)c->predrawn, (void *)c->d_order})
-        if (p) hipFree(p);
-    for (hipEvent_t e : c->events) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LYM_OK;
 }
@@ -425,7 +436,7 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     if (predrawn) {
         const size_t need = per_sweep * nsweeps;
         if (need > c->predrawn_cap) {
-            if (c->predrawn) hipFree(c->predrawn);
+            if (c->predrawn) (void)hipFree(c->predrawn);
             c->predrawn = nullptr;
             c->predrawn_cap = 0;
             LYM_HIP(c, hipMalloc(&c->predrawn, sizeof(double) * need));
@@ -514,7 +525,8 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
 int lym_set_schedule(lym_ctx *c, int kind, int target)
 {
     if (!c) return LYM_ERR_ARG;
-    if (kind < lym::SCHED_LINEAR || kind > lym::SCHED_COLUMN || (kind == lym::SCHED_COLUMN && target < 1))
+    const int order = kind & ~lym::SCHED_LDS;
+    if ((kind & ~(lym::SCHED_LDS | 3)) || order > lym::SCHED_COLUMN || (order == lym::SCHED_COLUMN && target < 1))
         return fail(c, LYM_ERR_ARG, "bad schedule");
     LYM_HIP(c, hipSetDevice(c->device));
     LYM_HIP(c, hipStreamSynchronize(c->stream));
@@ -551,13 +563,11 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
             for (int rep = 0; rep < 4; rep++) {  // stage 1 leaves W[cur] unchanged
                 LYM_HIP(c, hipEventRecord(c->events[0], c->stream));
                 if (c->prec == LYM_PREC_FP64)
-                    lym::k_flow_stage<double, 1, false><<<stencil_blocks(c), 256, 0, c->stream>>>(
-                        static_cast<double2 *>(c->W[c->cur]), static_cast<double2 *>(c->W[1 - c->cur]),
-                        static_cast<double *>(c->acc), c->g, dev_sched(c), 0.0);
+                    launch_stage<double, 1>(c, static_cast<double2 *>(c->W[c->cur]),
+                                            static_cast<double2 *>(c->W[1 - c->cur]), static_cast<double *>(c->acc), 0.0);
                 else
-                    lym::k_flow_stage<float, 1, true><<<stencil_blocks(c), 256, 0, c->stream>>>(
-                        static_cast<float2 *>(c->W[c->cur]), static_cast<float2 *>(c->W[1 - c->cur]),
-                        static_cast<float *>(c->acc), c->g, dev_sched(c), 0.0);
+                    launch_stage<float, 1>(c, static_cast<float2 *>(c->W[c->cur]), static_cast<float2 *>(c->W[1 - c->cur]),
+                                           static_cast<float *>(c->acc), 0.0);
                 LYM_LAUNCHED(c, "k_flow_stage");
                 LYM_HIP(c, hipEventRecord(c->events[1], c->stream));
                 LYM_HIP(c, hipEventSynchronize(c->events[1]));
@@ -589,32 +599,31 @@ int lym_copy_bandwidth(int device, int64_t bytes, int reps, double *gbs)
     if (bytes < 4096 || reps < 1 || !gbs) return LYM_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
-    if (hipSetDevice(device) != hipSuccess) return LYM_ERR_HIP;
+    if (device < 0 || device >= ndev || hipSetDevice(device) != hipSuccess) return LYM_ERR_ARG;
     const long n = long(bytes / 16);
     double2 *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, 16 * size_t(n)) != hipSuccess) return LYM_ERR_NOMEM;
-    if (hipMalloc(&b, 16 * size_t(n)) != hipSuccess) {
-        hipFree(a);
-        return LYM_ERR_NOMEM;
-    }
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    const unsigned nb = unsigned(std::min<long>((n + 255) / 256, 1L << 20));
-    lym::k_stream_copy<<<nb, 256>>>(a, b, n);  // first touch
-    hipEventRecord(e0, nullptr);
-    for (int r = 0; r < reps; r++) lym::k_stream_copy<<<nb, 256>>>(a, b, n);
-    hipEventRecord(e1, nullptr);
-    hipError_t e = hipEventSynchronize(e1);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = LYM_OK;
+    if (hipMalloc(&a, 16 * size_t(n)) != hipSuccess || hipMalloc(&b, 16 * size_t(n)) != hipSuccess)
+        rc = LYM_ERR_NOMEM;
+    else if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = LYM_ERR_HIP;
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipFree(a);
-    hipFree(b);
-    if (e != hipSuccess) return LYM_ERR_HIP;
-    *gbs = 2.0 * 16.0 * double(n) * reps / (ms * 1e-3) / 1e9;
-    return LYM_OK;
+    if (rc == LYM_OK) {
+        const unsigned nb = unsigned(std::min<long>((n + 255) / 256, 1L << 20));
+        lym::k_stream_copy<<<nb, 256>>>(a, b, n);  // first touch
+        bool ok = hipEventRecord(e0, nullptr) == hipSuccess;
+        for (int r = 0; r < reps; r++) lym::k_stream_copy<<<nb, 256>>>(a, b, n);
+        ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, nullptr) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        if (!ok) rc = LYM_ERR_HIP;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (rc == LYM_OK) *gbs = 2.0 * 16.0 * double(n) * reps / (ms * 1e-3) / 1e9;
+    return rc;
 }
 
 }  // extern "C"

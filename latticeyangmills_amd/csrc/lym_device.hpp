@@ -372,19 +372,20 @@ LYM_DI su3<double> exp_assemble(const herm8<double> &q, const su3<double> &q2, d
 // needs no transcendental function.
 constexpr double kSeriesC1Max = 0.3;
 constexpr int kSeriesTerms = 18;
+// Short tier: c1 <= 0.004 (spectral radius <= 0.073) needs 9 terms for a
+// truncation error below 2e-18.  The Wilson flow at eps <= 0.02 is almost
+// always there; the tier is chosen per wave (all lanes qualify) so that the
+// wave never runs both loops.
+constexpr double kShortSeriesC1Max = 0.004;
+constexpr int kShortSeriesTerms = 9;
 
-LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, double d2, double c1)
+template <int N>
+LYM_DI void series_coeffs(double c0, double c1, cplx<double> &f0, cplx<double> &f1, cplx<double> &f2)
 {
-    cplx<double> p = cfma(q.o01, q.o12, cplx<double>{0.0, 0.0});
-    const double re3 = p.re * q.o02.re + p.im * q.o02.im;
-    const double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
-    const double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
-    const double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
-    const double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
     double a = 1.0, b = 0.0, d = 0.0, inv_fact = 1.0;
     double f0r = 0, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
 #pragma unroll
-    for (int n = 0; n <= kSeriesTerms; n++) {
+    for (int n = 0; n <= N; n++) {
         if (n > 0) inv_fact *= 1.0 / double(n);
         const double ta = a * inv_fact, tb = b * inv_fact, td = d * inv_fact;
         switch (n & 3) {  // i^n = 1, i, -1, -i
@@ -398,7 +399,25 @@ LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, 
         a = an;
         b = bn;
     }
-    return exp_assemble(q, q2, d2, cplx<double>{f0r, f0i}, cplx<double>{f1r, f1i}, cplx<double>{f2r, f2i});
+    f0 = cplx<double>{f0r, f0i};
+    f1 = cplx<double>{f1r, f1i};
+    f2 = cplx<double>{f2r, f2i};
+}
+
+LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, double d2, double c1)
+{
+    cplx<double> p = cfma(q.o01, q.o12, cplx<double>{0.0, 0.0});
+    const double re3 = p.re * q.o02.re + p.im * q.o02.im;
+    const double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
+    const double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
+    const double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
+    const double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
+    cplx<double> f0, f1, f2;
+    if (__all(c1 <= kShortSeriesC1Max))
+        series_coeffs<kShortSeriesTerms>(c0, c1, f0, f1, f2);
+    else
+        series_coeffs<kSeriesTerms>(c0, c1, f0, f1, f2);
+    return exp_assemble(q, q2, d2, f0, f1, f2);
 }
 
 // exp(iQ) for Hermitian traceless Q: Cayley-Hamilton form of Morningstar &

@@ -27,38 +27,99 @@ LYM_DI long sched_run(const Sched sc, unsigned b)
 
 template <typename S> using F2 = vec2_t<S>;
 
+// ---------------------------------------------------------- link sources ---
+
+// Where a link read of the staple lies relative to the workgroup's 64-site
+// run, known at compile time for each term:
+enum LinkWhere { ELSEWHERE = 0, XAXIS = 1, OWN = 2 };
+//   OWN       the site itself (always inside the run)
+//   XAXIS     a neighbour along x only (inside the run when runs hold whole
+//             x-rows, i.e. Lx divides 64)
+//   ELSEWHERE any other neighbour
+
+// Links of a field read straight from HBM (through L1/L2).
+template <typename S> struct GlobalLinks {
+    const F2<S> *f;
+    long V;
+    template <typename R, int WHERE> LYM_DI su3<R> get(int dir, long site) const
+    {
+        return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+    }
+};
+
+// The four links of the run's 64 sites staged in LDS:
+// lds[(dir*9 + k)*64 + (site - site0)], consecutive lanes at consecutive
+// addresses (no bank conflicts).  OWN reads, and XAXIS reads when XROWS, come
+// from LDS; the choice is made at compile time, so no lane ever diverges.
+// Per link update 3 of the 19 staple reads are OWN and, with XROWS, another
+// 3 are XAXIS: a third of the vector-memory traffic moves to LDS.
+template <typename S, bool XROWS> struct RunLinks {
+    const F2<S> *f;
+    long V;
+    long site0;
+    const F2<S> *lds;
+    template <typename R, int WHERE> LYM_DI su3<R> get(int dir, long site) const
+    {
+        if constexpr (WHERE == OWN || (WHERE == XAXIS && XROWS)) {
+            const unsigned li = unsigned(site - site0);
+            su3<R> m;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const F2<S> v = lds[(dir * 9 + k) * kSitesPerBlock + li];
+                m.e[k] = cplx<R>{R(v.x), R(v.y)};
+            }
+            return m;
+        } else {
+            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+        }
+    }
+};
+
 // ---------------------------------------------------------------- staple ---
+
+// The two staples of plane (MU, NU) for link U_MU(x):
+//   U_NU(x+MU) U_MU(x+NU)^+ U_NU(x)^+  +  U_NU(x+MU-NU)^+ U_MU(x-NU)^+ U_NU(x-NU)
+template <typename R, int MU, int NU, typename Src>
+LYM_DI void staple_plane(const Src &src, long s, const Nbr &n, su3<R> &om)
+{
+    constexpr int AMU = (MU == 0) ? XAXIS : ELSEWHERE;  // x + MU
+    constexpr int ANU = (NU == 0) ? XAXIS : ELSEWHERE;  // x + NU, x - NU
+    {
+        su3<R> a = src.template get<R, AMU>(NU, s + n.fwd[MU]);
+        su3<R> b = src.template get<R, ANU>(MU, s + n.fwd[NU]);
+        su3<R> t = mul_na(a, b);
+        su3<R> c = src.template get<R, OWN>(NU, s);
+        acc_na(om, t, c);
+    }
+    {
+        const long y = s + n.bwd[NU];  // same MU-coordinate as s
+        su3<R> a = src.template get<R, ANU>(MU, y);
+        su3<R> b = src.template get<R, ELSEWHERE>(NU, y + n.fwd[MU]);
+        su3<R> t = mul_nn(a, b);
+        su3<R> c = src.template get<R, ANU>(NU, y);
+        acc_an(om, t, c);
+    }
+}
 
 // Omega_mu(x) = sum_{nu != mu} [ U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
 //                              + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu) ]
 // (src/Actions/puregauge.cpp:36-54), so that U_mu(x) Omega_mu(x) is the sum
 // of the six plaquettes through the link.
+template <typename R, int MU, typename Src>
+LYM_DI su3<R> staple_from(const Src &src, long s, const Nbr &n)
+{
+    su3<R> om = zero3<R>();
+    if constexpr (MU != 0) staple_plane<R, MU, 0>(src, s, n, om);
+    if constexpr (MU != 1) staple_plane<R, MU, 1>(src, s, n, om);
+    if constexpr (MU != 2) staple_plane<R, MU, 2>(src, s, n, om);
+    if constexpr (MU != 3) staple_plane<R, MU, 3>(src, s, n, om);
+    return om;
+}
+
 template <typename R, typename S, int MU>
 LYM_DI su3<R> staple(const F2<S> *__restrict__ f, long V, long s, const Nbr &n)
 {
-    const F2<S> *Fmu = f + (long)MU * 9 * V;
-    su3<R> om = zero3<R>();
-#pragma unroll
-    for (int nu = 0; nu < 4; nu++) {
-        if (nu == MU) continue;
-        const F2<S> *Fnu = f + (long)nu * 9 * V;
-        {
-            su3<R> a = load_link<R, S>(Fnu, V, s + n.fwd[MU]);
-            su3<R> b = load_link<R, S>(Fmu, V, s + n.fwd[nu]);
-            su3<R> t = mul_na(a, b);
-            su3<R> c = load_link<R, S>(Fnu, V, s);
-            acc_na(om, t, c);
-        }
-        {
-            const long y = s + n.bwd[nu];  // same mu-coordinate as s
-            su3<R> a = load_link<R, S>(Fmu, V, y);
-            su3<R> b = load_link<R, S>(Fnu, V, y + n.fwd[MU]);
-            su3<R> t = mul_nn(a, b);
-            su3<R> c = load_link<R, S>(Fnu, V, y);
-            acc_an(om, t, c);
-        }
-    }
-    return om;
+    return staple_from<R, MU>(GlobalLinks<S>{f, V}, s, n);
 }
 
 // ---------------------------------------------------- flow: one RK stage ---
@@ -72,58 +133,92 @@ LYM_DI su3<R> staple(const F2<S> *__restrict__ f, long V, long s, const Nbr &n)
 // before any update).  Staple and force run in the storage precision S;
 // the RK combination, the exponential and exp*U run in FP64.  REUNIT
 // projects the result back onto SU(3) before it is stored (mixed path).
-template <typename S, int STAGE, bool REUNIT, int MU>
-LYM_DI void flow_link(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
-                      const Geom g, long s, double eps)
+template <typename S, int STAGE, bool REUNIT, int MU, typename Src>
+LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ acc, const Geom g, long s,
+                      double eps)
 {
     const long V = g.V;
     Nbr n = neighbours(g, s);
-    su3<S> om = staple<S, S, MU>(win, V, s, n);
-    su3<S> U = load_link<S, S>(win + (long)MU * 9 * V, V, s);
+    su3<S> om = staple_from<S, MU>(src, s, n);
+    su3<S> U = src.template get<S, OWN>(MU, s);
     su3<S> X = mul_nn(U, om);
     herm8<double> z = force_from_x(X);
     S *am = acc + (long)MU * 8 * V;
     herm8<double> q;
     if constexpr (STAGE == 1) {
-        store_h8<S>(am, V, s, z);
+        store_h8<S>(am, V, unsigned(s), z);
         q = h8_scale(eps * 0.25, z);
     } else if constexpr (STAGE == 2) {
-        herm8<double> a = load_h8<S>(am, V, s);
+        herm8<double> a = load_h8<S>(am, V, unsigned(s));
         a = h8_axpby(8.0 / 9.0, z, -17.0 / 36.0, a);
-        store_h8<S>(am, V, s, a);
+        store_h8<S>(am, V, unsigned(s), a);
         q = h8_scale(eps, a);
     } else {
-        herm8<double> a = load_h8<S>(am, V, s);
+        herm8<double> a = load_h8<S>(am, V, unsigned(s));
         q = h8_scale(eps, h8_axpby(0.75, z, -1.0, a));
     }
     su3<double> E = exp_iq(q);
     su3<double> W = mul_nn(E, convert<double>(U));
     if constexpr (REUNIT) reunitarize(W);
-    store_link<S, double>(wout + (long)MU * 9 * V, V, s, W);
+    store_link<S, double>(wout + (long)MU * 9 * V, V, unsigned(s), W);
 }
 
-template <typename S, int STAGE, bool REUNIT>
+// One 64-site run, all four directions (wave = direction).  LDS != 0 stages
+// the run's own links in LDS first (LDS == 2: runs hold whole x-rows, so
+// x-axis neighbours are read from LDS too) (every wave loads its direction's 64 links
+// and shares them); all 256 threads reach the barrier, then lanes past the end
+// of the lattice drop out.  The staging buffer is free again when the
+// function returns (trailing barrier), so a persistent caller may loop.
+template <typename S, int STAGE, bool REUNIT, int LDS>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
                        const Geom g, long site0, double eps)
 {
     const int mu = threadIdx.x >> 6;
-    const long s = site0 + (threadIdx.x & 63);
-    if (s >= g.V) return;
-    switch (mu) {
-    case 0: flow_link<S, STAGE, REUNIT, 0>(win, wout, acc, g, s, eps); break;
-    case 1: flow_link<S, STAGE, REUNIT, 1>(win, wout, acc, g, s, eps); break;
-    case 2: flow_link<S, STAGE, REUNIT, 2>(win, wout, acc, g, s, eps); break;
-    default: flow_link<S, STAGE, REUNIT, 3>(win, wout, acc, g, s, eps); break;
+    const int lane = threadIdx.x & 63;
+    const long s = site0 + lane;
+    const bool active = s < g.V;
+    if constexpr (LDS != 0) {
+        __shared__ F2<S> lds[4 * 9 * kSitesPerBlock];
+        if (active) {
+            const su3<S> u = load_link<S, S>(win + (long)mu * 9 * g.V, g.V, unsigned(s));
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                F2<S> v;
+                v.x = u.e[k].re;
+                v.y = u.e[k].im;
+                lds[(mu * 9 + k) * kSitesPerBlock + lane] = v;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const RunLinks<S, LDS == 2> src{win, g.V, site0, lds};
+            switch (mu) {
+            case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
+            case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
+            case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
+            default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+            }
+        }
+        __syncthreads();
+    } else {
+        if (!active) return;
+        const GlobalLinks<S> src{win, g.V};
+        switch (mu) {
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        }
     }
 }
 
-template <typename S, int STAGE, bool REUNIT>
+template <typename S, int STAGE, bool REUNIT, int LDS>
 __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                     S *__restrict__ acc, Geom g, Sched sc, double eps)
 {
-    const long run = sched_run(sc, blockIdx.x);
+    const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
     if (run < 0) return;
-    flow_block<S, STAGE, REUNIT>(win, wout, acc, g, run * kSitesPerBlock, eps);
+    flow_block<S, STAGE, REUNIT, LDS>(win, wout, acc, g, run * kSitesPerBlock, eps);
 }
 
 // ----------------------------------------------------------- observables ---

@@ -24,6 +24,8 @@
 namespace lym {
 
 enum SchedKind { SCHED_LINEAR = 0, SCHED_CONTIG = 1, SCHED_COLUMN = 2 };
+// flag or-ed into a kind: the stage kernel stages each run's own links in LDS
+constexpr int SCHED_LDS = 16;
 constexpr int kXcds = 8;
 constexpr int kRunSites = 64;
 
@@ -71,6 +73,7 @@ inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, 
     HostSchedule h;
     h.kind = kind;
     h.target = target;
+    kind &= ~SCHED_LDS;
     if (kind == SCHED_LINEAR) return h;
     std::vector<int> all;
     for (size_t c = 0; c < dims.size(); c++) {
@@ -92,16 +95,31 @@ inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, 
     return h;
 }
 
-// the candidates the autotuners time
+// the candidates the autotuners time: every order, with and without LDS staging
 inline std::vector<std::pair<int, int>> schedule_candidates()
 {
-    return {{SCHED_LINEAR, 0},     {SCHED_CONTIG, 0},     {SCHED_COLUMN, 1024},
-            {SCHED_COLUMN, 2048},  {SCHED_COLUMN, 4096},  {SCHED_COLUMN, 8192}};
+    std::vector<std::pair<int, int>> c;
+    for (int lds : {0, SCHED_LDS})
+        for (auto k : std::vector<std::pair<int, int>>{{SCHED_LINEAR, 0},
+                                                       {SCHED_CONTIG, 0},
+                                                       {SCHED_COLUMN, 1024},
+                                                       {SCHED_COLUMN, 2048},
+                                                       {SCHED_COLUMN, 4096},
+                                                       {SCHED_COLUMN, 8192}})
+            c.push_back({k.first | lds, k.second});
+    return c;
 }
 
 inline const char *schedule_name(int kind)
 {
-    return kind == SCHED_LINEAR ? "linear" : kind == SCHED_CONTIG ? "contig" : "column";
+    switch (kind) {
+    case SCHED_LINEAR: return "linear";
+    case SCHED_CONTIG: return "contig";
+    case SCHED_COLUMN: return "column";
+    case SCHED_LINEAR | SCHED_LDS: return "linear+lds";
+    case SCHED_CONTIG | SCHED_LDS: return "contig+lds";
+    default: return "column+lds";
+    }
 }
 
 }  // namespace lym

@@ -52,19 +52,23 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
     return lo;
 }
 
-template <typename S, int STAGE, bool REUNIT, int LDS>
+// PERSIST = false: one workgroup per schedule slot (no loop, so nothing is
+// kept live across tiles); true: a persistent grid strides over the slots.
+template <typename S, int STAGE, bool REUNIT, int LDS, bool PERSIST>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
                                                           int nconf, long ntiles, Sched sc, double eps)
 {
     const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
     for (long k = blockIdx.x; k < slots; k += gridDim.x) {
-        const long tile = sched_run(sc, unsigned(k));
-        if (tile < 0) continue;
-        const int c = find_conf(desc, nconf, tile);
-        const ConfDesc d = desc[c];
-        flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
-                                     (tile - d.tile0) * kSitesPerBlock, eps);
+        const long tile = sched_run(sc, unsigned(k));  // uniform over the workgroup
+        if (tile >= 0) {
+            const int c = find_conf(desc, nconf, tile);
+            const ConfDesc d = desc[c];
+            flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
+                                              (tile - d.tile0) * kSitesPerBlock, eps);
+        }
+        if constexpr (!PERSIST) break;
     }
 }
 
@@ -242,15 +246,17 @@ void launch_stage(const lym_batch *b, const lym::F2<S> *win, lym::F2<S> *wout, S
     const unsigned g = stage_grid(b);
     bool xrows = true;  // every lattice's runs hold whole x-rows
     for (auto &d : b->dims) xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
-    if (!(b->sched.kind & lym::SCHED_LDS))
-        lym::k_batch_flow_stage<S, STAGE, re, 0><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
-                                                                          b->ntiles, dev_sched(b), eps);
-    else if (xrows)
-        lym::k_batch_flow_stage<S, STAGE, re, 2><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
-                                                                          b->ntiles, dev_sched(b), eps);
-    else
-        lym::k_batch_flow_stage<S, STAGE, re, 1><<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf,
-                                                                          b->ntiles, dev_sched(b), eps);
+    const int lds = !(b->sched.kind & lym::SCHED_LDS) ? 0 : xrows ? 2 : 1;
+    auto go = [&](auto kern) { kern<<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf, b->ntiles, dev_sched(b), eps); };
+    if (b->grid > 0) {
+        if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, true>);
+        else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, true>);
+        else go(lym::k_batch_flow_stage<S, STAGE, re, 1, true>);
+    } else {
+        if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, false>);
+        else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, false>);
+        else go(lym::k_batch_flow_stage<S, STAGE, re, 1, false>);
+    }
 }
 
 template <typename S> int enqueue_step(lym_batch *b, double eps)
@@ -398,7 +404,7 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(ConfDesc) * nconf, hipMemcpyHostToDevice) != hipSuccess)
         return bail(LYM_ERR_HIP);
     if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
-    if (apply_schedule(b, 0, lym::SCHED_COLUMN, precision == LYM_PREC_FP64 ? 2048 : 4096) != LYM_OK)
+    if (apply_schedule(b, 0, lym::SCHED_COLUMN | lym::SCHED_LDS, 8192) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
     // cold start
     for (int c = 0; c < nconf; c++) {

@@ -232,7 +232,7 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
     if (hipMalloc(&c->staging, sizeof(double) * 18 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
-    if (apply_schedule(c, lym::SCHED_COLUMN, precision == LYM_PREC_FP64 ? 2048 : 4096) != LYM_OK)
+    if (apply_schedule(c, lym::SCHED_COLUMN | lym::SCHED_LDS, 8192) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
     int rc = lym_set_cold(c);
     if (rc != LYM_OK) return bail(rc);

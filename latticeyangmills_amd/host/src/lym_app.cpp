@@ -5,6 +5,10 @@
 //                [--precision fp64|mixed] [--measure-every K]
 //   lym_app generate --lattice Lx Ly Lz Lt --beta B --eps E --therm N --steps M
 //                --confs K [--start Cold|Hot] [--seed S] [--nhit H] --output DIR
+//   lym_app run INPUT.json     (the reference's own input files: App FLOW or
+//                              PGFG, LatticeSize, Action puregauge, Observables,
+//                              InputDir, OutputDir; optional build keys
+//                              "precision", "measureEvery", "seed", "nHit")
 //   lym_app io-selftest DIR   (host only: writes and reads back a .bin file)
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +20,7 @@
 #include "Apps/gpugaugefieldfactory.h"
 #include "Apps/gpuwilsonflow.h"
 #include "InputOutput/confio.h"
+#include "InputOutput/jsonlite.h"
 #include "Observables/gpusuperobs.h"
 
 namespace {
@@ -66,11 +71,69 @@ bool parse(int argc, char **argv, Args &a)
 }
 }  // namespace
 
+// JSON input of the reference's front-end (src/InputOutput/JsonInput/JsonGeneric/jsonapp.cpp:49-89,
+// jsonlattice.cpp:11-73, JsonAction/jsonaction.cpp:60-89, JsonObservable/jsonobservable.cpp:41-78,
+// jsondirectories.cpp:15-55).  SubLatticeSize must equal LatticeSize (one GPU holds the whole volume).
+int run_json(const char *path)
+{
+    using LatticeIO::JsonValue;
+    const JsonValue j = JsonValue::parseFile(path);
+    const JsonValue &ls = j["LatticeSize"];
+    const std::array<int, 4> lat{{ls["x"].integer(), ls["y"].integer(), ls["z"].integer(), ls["t"].integer()}};
+    if (j.has("SubLatticeSize")) {
+        const JsonValue &ss = j["SubLatticeSize"];
+        const std::array<int, 4> sub{{ss["x"].integer(), ss["y"].integer(), ss["z"].integer(), ss["t"].integer()}};
+        if (sub != lat) throw std::runtime_error("SubLatticeSize must equal LatticeSize: one GPU holds the whole lattice");
+    }
+    if (j["Action"]["type"].string() != "puregauge") throw std::runtime_error("only the puregauge action exists");
+    const double beta = j["Action"]["params"]["beta"].number();
+    const int precision = (j.has("precision") && j["precision"].string() == "mixed") ? LYM_PREC_MIXED : LYM_PREC_FP64;
+    const std::string out = j.has("OutputDir") ? j["OutputDir"].string() : ".";
+    const std::string type = j["App"]["type"].string();
+    const JsonValue &p = j["App"]["params"];
+    bool superobs = false;
+    if (j.has("Observables"))
+        for (auto &o : j["Observables"].arr) superobs = superobs || o.string() == "superobs";
+    GpuSuperObs obs(precision);
+    GpuPureGauge act(beta, precision);
+    if (type == "FLOW") {
+        GpuWilsonFlow app(p["tauFinal"].number(), p["epsilon"].number(), precision,
+                          j.has("measureEvery") ? j["measureEvery"].integer() : 1);
+        app.setAction(&act);
+        if (superobs) app.addObservable(&obs);
+        app.createLattice(lat);
+        app.setInputDir(j["InputDir"].string());
+        app.setOutputDir(out);
+        app.execute();
+        printf("flowed %zu configuration(s) to t=%g\n", app.inputList().size(), p["tauFinal"].number());
+        return 0;
+    }
+    if (type == "PGFG") {
+        const int mc = p["MCSteps"].integer();
+        const int nconf = p.has("NConf") ? p["NConf"].integer() : mc / p["NCorr"].integer();
+        GpuGaugeFieldFactory app(mc, p["thermSteps"].integer(), nconf, p["SU3eps"].number(), p["startType"].string(),
+                                 beta, j.has("seed") ? (unsigned long long)j["seed"].number() : 0ull,
+                                 j.has("nHit") ? j["nHit"].integer() : 10);
+        app.setAction(&act);
+        if (superobs) app.addObservable(&obs);
+        app.createLattice(lat);
+        app.setOutputDir(out);
+        app.execute();
+        printf("generated %d configuration(s), acceptance %.4f\n", nconf, app.acceptance());
+        return 0;
+    }
+    throw std::runtime_error("unsupported App type " + type + " (FLOW and PGFG are built)");
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) return usage();
     const std::string cmd = argv[1];
     try {
+        if (cmd == "run") {
+            if (argc < 3) return usage();
+            return run_json(argv[2]);
+        }
         if (cmd == "io-selftest") {
             if (argc < 3) return usage();
             const std::array<int, 4> dims{{2, 4, 2, 4}};

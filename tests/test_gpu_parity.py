@@ -182,14 +182,17 @@ def test_mixed_precision_against_fp64(therm8):
         assert abs(got[r, 2] - want[r, 2]) < 1e-4, r
 
 
-@pytest.mark.parametrize("kind,target", [("linear", 0), ("contig", 0), ("column", 64), ("column", 300)])
-def test_schedule_does_not_change_results(therm8, kind, target):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind,target,lds", [("linear", 0, False), ("contig", 0, False), ("column", 64, False),
+                                             ("column", 300, False), ("linear", 0, True), ("column", 300, True)])
+def test_schedule_does_not_change_results(therm8, precision, kind, target, lds):
     dims, links = therm8
-    with LatticeContext(dims, "mixed") as ctx:
+    with LatticeContext(dims, precision) as ctx:
+        ctx.set_schedule("linear", 0)
         ctx.upload(links)
         ref = ctx.flow(0.02, 0.2, 3)
         ref_links = ctx.download()
-        ctx.set_schedule(kind, target)
+        ctx.set_schedule(kind, target, lds)
         ctx.upload(links)
         got = ctx.flow(0.02, 0.2, 3)
         assert np.array_equal(ctx.download(), ref_links)
@@ -205,3 +208,15 @@ def test_autotune_picks_a_schedule(therm8, tmp_path):
         assert "schedule=" in s and "measured" in s
         assert "cached" in ctx.autotune(str(tmp_path / "tune.txt"))
         assert np.array_equal(ctx.download(), links)  # tuning leaves the field alone
+
+
+@pytest.mark.parametrize("dims", [(6, 4, 4, 8), (12, 4, 2, 4)])
+def test_lds_variant_without_whole_rows(dims):
+    """Lx not dividing 64: only own-site links come from LDS; results unchanged."""
+    links = orc.hot_start(dims, 31)
+    want = orc.flow_step(links, dims, 0.02)
+    with LatticeContext(dims) as ctx:
+        ctx.set_schedule("contig", 0, True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        assert np.max(np.abs(ctx.download() - want)) < 1e-12

@@ -57,3 +57,43 @@ def test_generate_then_flow(tmp_path):
         assert np.all(np.abs(got[:, 1] / want[:, 1] - 1) < 1e-12)
         assert np.all(np.abs(got[:, 3] / want[:, 3] - 1) < 1e-12)
         assert np.all(np.abs(got[:, 2] - want[:, 2]) < 1e-10 * scale)
+
+
+def run_rc(*args):
+    return subprocess.run([APP, *map(str, args)], capture_output=True, text=True, timeout=120)
+
+
+def test_json_front_end_rejects_bad_input(tmp_path):
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"App": {"type": "FLOW", "params": {"epsilon": 0.01, "tauFinal": 1.0}}, "LatticeSize": {"x": 4,')
+    r = run_rc("run", bad)
+    assert r.returncode == 1 and "JSON:" in r.stderr
+    sub = tmp_path / "sub.json"
+    sub.write_text('{"App": {"type": "FLOW", "params": {"epsilon": 0.01, "tauFinal": 1.0}},'
+                   '"LatticeSize": {"x": 4, "y": 4, "z": 4, "t": 8}, "SubLatticeSize": {"x": 4, "y": 4, "z": 4, "t": 4},'
+                   '"Action": {"type": "puregauge", "params": {"beta": 6.0}}, "Observables": ["superobs"],'
+                   '"InputDir": ".", "OutputDir": "."}')
+    r = run_rc("run", sub)
+    assert r.returncode == 1 and "SubLatticeSize" in r.stderr
+
+
+@pytest.mark.gpu
+def test_json_front_end_pgfg_then_flow(tmp_path):
+    """The reference's own input format (jsonapp.cpp:49-89): PGFG writes configs, FLOW flows them."""
+    import json
+    confs, out = tmp_path / "confs", tmp_path / "out"
+    common = {"LatticeSize": {"x": 4, "y": 4, "z": 4, "t": 8}, "SubLatticeSize": {"x": 4, "y": 4, "z": 4, "t": 8},
+              "Action": {"type": "puregauge", "params": {"beta": 6.0}}, "Observables": ["superobs"]}
+    pgfg = dict(common, App={"type": "PGFG", "params": {"MCSteps": 4, "thermSteps": 5, "startType": "Hot",
+                                                        "NConf": 2, "SU3eps": 0.24}}, OutputDir=str(confs), seed=3)
+    (tmp_path / "pgfg.json").write_text(json.dumps(pgfg))
+    assert "generated 2" in run("run", tmp_path / "pgfg.json")
+    flow = dict(common, App={"type": "FLOW", "params": {"epsilon": 0.02, "tauFinal": 0.2}},
+                InputDir=str(confs), OutputDir=str(out))
+    (tmp_path / "flow.json").write_text(json.dumps(flow))
+    assert "flowed 2" in run("run", tmp_path / "flow.json")
+    dims = (4, 4, 4, 8)
+    got = np.loadtxt(out / "flow" / "conf0001.dat")
+    want, _ = orc.flow(read_bin(confs / "conf0001.bin", dims), dims, 0.02, 0.2, 1, 0)
+    assert np.all(np.abs(got[:, 1] / want[:, 1] - 1) < 1e-12)
+    assert np.all(np.abs(got[:, 3] / want[:, 3] - 1) < 1e-12)

@@ -358,16 +358,17 @@ __global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f
     block_ksum4<192>(v, partials + 4 * blockIdx.x);
 }
 
-// Sum nblk block partials into dst[0..3] (one workgroup).
-static __global__ __launch_bounds__(256) void k_reduce_partials(const ksum *__restrict__ partials, int nblk,
-                                                         double *__restrict__ dst)
+// Sum nblk block partials into dst[0..3] (one workgroup of 1024 threads).
+constexpr int kReduceThreads = 1024;
+static __global__ __launch_bounds__(kReduceThreads) void k_reduce_partials(const ksum *__restrict__ partials, int nblk,
+                                                                    double *__restrict__ dst)
 {
     ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    for (int i = threadIdx.x; i < nblk; i += 256)
+    for (int i = threadIdx.x; i < nblk; i += kReduceThreads)
 #pragma unroll
         for (int k = 0; k < 4; k++) v[k] = kadd(v[k], partials[4 * i + k]);
     __shared__ ksum tot[4];
-    block_ksum4<256>(v, tot);
+    block_ksum4<kReduceThreads>(v, tot);
     __syncthreads();
     if (threadIdx.x < 4) dst[threadIdx.x] = tot[threadIdx.x].s + tot[threadIdx.x].c;
 }

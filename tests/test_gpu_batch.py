@@ -57,10 +57,13 @@ def test_flow_matches_single_lattice_path(inputs, precision, grid, kind, target,
         series = b.flow(eps, tau, every, use_graph=False)
         for c in range(len(DIMS)):
             s, links = singles[c]
-            assert np.array_equal(b.download(c), links)
+            # The exp series length is chosen per wave (9 or 18 terms, both
+            # truncating below 2e-18), so another grouping of sites into waves
+            # may round a few links differently: rounding-level, not bitwise.
+            assert np.abs(b.download(c) - links).max() < (1e-13 if precision == "fp64" else 2e-6)
             m = ~np.isnan(s[:, 1])
             assert np.array_equal(np.isnan(series[c, :, 1]), ~m)
-            # same links; observables differ only by summation order (Q cancels: absolute bound)
+            # observables differ by that rounding and by summation order (Q cancels: absolute bound)
             assert np.allclose(series[c][m][:, [0, 1, 3]], s[m][:, [0, 1, 3]], rtol=1e-13, atol=0)
             assert np.allclose(series[c][m][:, 2], s[m][:, 2], rtol=0, atol=1e-11)
 

@@ -34,7 +34,8 @@ struct Args {
 
 int usage()
 {
-    fprintf(stderr, "usage: lym_app flow|generate|io-selftest ... (see lym_app.cpp)\n");
+    fprintf(stderr, "usage: lym_app run INPUT.json | flow ... | generate ... | io-selftest DIR\n"
+                    "       (options are listed at the top of lym_app.cpp)\n");
     return 2;
 }
 

@@ -60,12 +60,14 @@ def test_flow_matches_single_lattice_path(inputs, precision, grid, kind, target,
             # The exp series length is chosen per wave (9 or 18 terms, both
             # truncating below 2e-18), so another grouping of sites into waves
             # may round a few links differently: rounding-level, not bitwise.
-            assert np.abs(b.download(c) - links).max() < (1e-13 if precision == "fp64" else 2e-6)
+            # Mixed links are stored in FP32, so such a difference is one FP32 ulp.
+            fp64 = precision == "fp64"
+            assert np.abs(b.download(c) - links).max() < (1e-13 if fp64 else 2e-6)
             m = ~np.isnan(s[:, 1])
             assert np.array_equal(np.isnan(series[c, :, 1]), ~m)
             # observables differ by that rounding and by summation order (Q cancels: absolute bound)
-            assert np.allclose(series[c][m][:, [0, 1, 3]], s[m][:, [0, 1, 3]], rtol=1e-13, atol=0)
-            assert np.allclose(series[c][m][:, 2], s[m][:, 2], rtol=0, atol=1e-11)
+            assert np.allclose(series[c][m][:, [0, 1, 3]], s[m][:, [0, 1, 3]], rtol=1e-13 if fp64 else 1e-7, atol=0)
+            assert np.allclose(series[c][m][:, 2], s[m][:, 2], rtol=0, atol=1e-11 if fp64 else 1e-5)
 
 
 def test_graph_replays_match_eager(inputs):
@@ -120,5 +122,7 @@ def test_autotune_keeps_results(inputs, tmp_path):
         for c, L in enumerate(inputs):
             b.upload(c, L)
         got = b.flow(0.02, 0.1, 1, use_graph=True)
-        assert np.allclose(got, ref, rtol=1e-13, atol=0)
+        # the tuned schedule may group sites into waves differently (FP32 ulp level)
+        assert np.allclose(got[..., [0, 1, 3]], ref[..., [0, 1, 3]], rtol=1e-7, atol=0)
+        assert np.allclose(got[..., 2], ref[..., 2], rtol=0, atol=1e-5)
         assert "cached" in b.autotune(cache)

@@ -197,11 +197,12 @@ def test_schedule_does_not_change_results(therm8, precision, kind, target, lds):
         got = ctx.flow(0.02, 0.2, 3)
         # The exp series length is chosen per wave (9 or 18 terms, both below
         # 2e-18 truncation); another site-to-wave grouping may round differently.
-        tol = 1e-13 if precision == "fp64" else 2e-6
-        assert np.abs(ctx.download() - ref_links).max() < tol
+        # Mixed links are stored in FP32, so such a difference is one FP32 ulp.
+        fp64 = precision == "fp64"
+        assert np.abs(ctx.download() - ref_links).max() < (1e-13 if fp64 else 2e-6)
         m = ~np.isnan(ref[:, 1])
-        assert np.allclose(got[m][:, [0, 1, 3]], ref[m][:, [0, 1, 3]], rtol=tol, atol=0)
-        assert np.allclose(got[m][:, 2], ref[m][:, 2], rtol=0, atol=1e-11 if precision == "fp64" else 1e-5)
+        assert np.allclose(got[m][:, [0, 1, 3]], ref[m][:, [0, 1, 3]], rtol=1e-13 if fp64 else 1e-7, atol=0)
+        assert np.allclose(got[m][:, 2], ref[m][:, 2], rtol=0, atol=1e-11 if fp64 else 1e-5)
 
 
 def test_autotune_picks_a_schedule(therm8, tmp_path):

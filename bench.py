@@ -34,6 +34,12 @@ VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 78.6}  # non-packed FP32 issues at the
 REALS_PER_LINK_UPDATE = 140
 STAGE_REALS = (44, 52, 44)
 FLOP_PER_LINK_UPDATE = 3 * 3300  # per stage ~16.5 matmuls of 198 flop + exp
+# HBM-side bytes per k_flow_stage launch, averaged over the three RK stages,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this bench (fetch
+# scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
+# back); the run, its commands and the per-stage figures are in PROFILE_FILE.
+PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
+PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.3236e9, ("mixed-32x64", "fp64"): 8.5114e9}
 
 WORKLOADS = {
     "mixed-32x64": dict(dims=(32, 32, 32, 64), beta=6.2, sweeps=200, seed=62, eps=0.02, tau=6.0, every=10,
@@ -212,7 +218,9 @@ def main():
     st = r["stage_ms"]
     step_kernel_ms = float(st[0] + st[1] + st[2])
     bytes_step = 4 * V * REALS_PER_LINK_UPDATE * real
+    # per launch, averaged over the three stage launches of a step
     achieved_gbs = bytes_step / (step_kernel_ms * 1e-3) / 1e9
+    traffic = PROFILED_TRAFFIC.get((args.workload, main_prec))
     out = {
         "metric": METRIC, "value": lups, "unit": "link-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -222,7 +230,9 @@ def main():
                    "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "flow_t_max": tau - eps,
                    "measure_every": every, "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": PROFILE_FILE if traffic else None,
+                     "algorithmic_bytes_per_launch": bytes_step / 3,
                      "kernel": "k_flow_stage (RK stages 1+2+3 per step)",
                      "stage_ms": [float(x) for x in st[:3]], "observables_ms": float(st[3]),
                      "valu_tflops": 4 * V * FLOP_PER_LINK_UPDATE / (step_kernel_ms * 1e-3) / 1e12,

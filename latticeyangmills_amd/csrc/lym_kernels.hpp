@@ -79,11 +79,38 @@ template <typename S, bool XROWS> struct RunLinks {
 
 // The two staples of plane (MU, NU) for link U_MU(x):
 //   U_NU(x+MU) U_MU(x+NU)^+ U_NU(x)^+  +  U_NU(x+MU-NU)^+ U_MU(x-NU)^+ U_NU(x-NU)
+// In FP32 (the mixed path, whose links are reunitarised after every stage)
+// each staple is formed from its first two rows and completed by unitarity:
+// 186 instead of 216 arithmetic operations per staple, with a deviation of
+// the order of the links' own distance from SU(3) (FP32 rounding).  The FP64
+// path multiplies out all three rows as the reference does.
 template <typename R, int MU, int NU, typename Src>
 LYM_DI void staple_plane(const Src &src, long s, const Nbr &n, su3<R> &om)
 {
     constexpr int AMU = (MU == 0) ? XAXIS : ELSEWHERE;  // x + MU
     constexpr int ANU = (NU == 0) ? XAXIS : ELSEWHERE;  // x + NU, x - NU
+    if constexpr (sizeof(R) == 4) {
+        {
+            su3<R> a = src.template get<R, AMU>(NU, s + n.fwd[MU]);
+            su3<R> b = src.template get<R, ANU>(MU, s + n.fwd[NU]);
+            su3<R> t, u;
+            mul_na_rows01(t, a, b);
+            su3<R> c = src.template get<R, OWN>(NU, s);
+            mul_na_rows01(u, t, c);
+            acc_su3_rows01(om, u);
+        }
+        {
+            const long y = s + n.bwd[NU];
+            su3<R> a = src.template get<R, ANU>(MU, y);
+            su3<R> b = src.template get<R, ELSEWHERE>(NU, y + n.fwd[MU]);
+            su3<R> t, u;
+            mul_nn_cols01(t, a, b);  // (a b)^+ rows 0, 1 need columns 0, 1 of a b
+            su3<R> c = src.template get<R, ANU>(NU, y);
+            mul_an_rows01(u, t, c);
+            acc_su3_rows01(om, u);
+        }
+        return;
+    }
     {
         su3<R> a = src.template get<R, AMU>(NU, s + n.fwd[MU]);
         su3<R> b = src.template get<R, ANU>(MU, s + n.fwd[NU]);

@@ -120,7 +120,9 @@ def batched_ensemble(tune_cache, every=10):
     Haar hot starts with seeds 0..63, 100 Metropolis sweeps, then the mixed
     flow eps = 0.01 to t = 2.0 (199 steps, observables every `every` steps) as
     one persistent launch per RK stage, graph-captured, autotuned; against the
-    64 lattices flowed one after another by the plain FP64 single-lattice path."""
+    64 lattices flowed one after another by the plain FP64 single-lattice path,
+    and (to separate the batching gain from the precision gain) by the mixed
+    single-lattice path."""
     from latticeyangmills_amd import LatticeContext
     from latticeyangmills_amd.batch import BatchContext
     eps, tau = 0.01, 2.0
@@ -139,22 +141,28 @@ def batched_ensemble(tune_cache, every=10):
         t0 = time.perf_counter()
         series_b = b.flow(eps, tau, every, use_graph=True)
         t_batch = time.perf_counter() - t0
-    ctxs = []
-    for c, L in enumerate(confs):
-        ctx = LatticeContext(BATCH_DIMS[c], "fp64")
-        ctx.upload(L)
-        ctxs.append(ctx)
-    ctxs[0].flow(eps, 0.05, every)  # warm the code objects
-    ctxs[0].upload(confs[0])
-    t0 = time.perf_counter()
-    series_s = [ctx.flow(eps, tau, every) for ctx in ctxs]
-    t_seq = time.perf_counter() - t0
-    for ctx in ctxs:
-        ctx.close()
+    def sequential(precision):
+        ctxs = []
+        for c, L in enumerate(confs):
+            ctx = LatticeContext(BATCH_DIMS[c], precision)
+            ctx.upload(L)
+            ctxs.append(ctx)
+        ctxs[0].flow(eps, 0.05, every)  # warm the code objects
+        ctxs[0].upload(confs[0])
+        t0 = time.perf_counter()
+        series = [ctx.flow(eps, tau, every) for ctx in ctxs]
+        el = time.perf_counter() - t0
+        for ctx in ctxs:
+            ctx.close()
+        return series, el
+
+    series_s, t_seq = sequential("fp64")
+    _, t_seq_mixed = sequential("mixed")
     e_err = max(float(np.nanmax(np.abs(series_b[c][:, 3] / series_s[c][:, 3] - 1))) for c in range(len(confs)))
     n = len(BATCH_DIMS)
     return {"configs_per_s": n / t_batch, "seconds": t_batch, "sequential_plain_fp64_configs_per_s": n / t_seq,
             "sequential_seconds": t_seq, "speedup": t_seq / t_batch, "tuner": tuner,
+            "sequential_mixed_configs_per_s": n / t_seq_mixed, "speedup_vs_sequential_mixed": t_seq_mixed / t_batch,
             "max_rel_err_E_vs_fp64": e_err,
             "workload": "64 lattices: 16 x 8^4, 16 x 12^4, 16 x 16^4, 16 x 16^3x32; beta 6.0; Haar seeds 0..63; "
                         f"100 sweeps; mixed flow eps {eps} to t {tau} (199 steps), observables every {every}"}

@@ -120,8 +120,8 @@ static __global__ __launch_bounds__(256) void k_batch_reduce(const ksum *__restr
 
 // checkerboard Metropolis over the batch: work item i of sub-sweep (mu, parity)
 // is half-site i - half0[c] of lattice c
-template <typename S>
-__global__ __launch_bounds__(256) void k_batch_metropolis(F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
+template <typename S, int CAND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_batch_metropolis(F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
                                                           int nconf, long nhalf, int mu, MetroParams p,
                                                           unsigned long long *__restrict__ accepted)
 {
@@ -142,12 +142,7 @@ __global__ __launch_bounds__(256) void k_batch_metropolis(F2<S> *__restrict__ f,
         q.seed = d.seed;
         const long s = parity_site(d.g, i - d.half0, p.parity);
         F2<S> *fc = f + d.link_off;
-        switch (mu) {
-        case 0: acc = metro_link<S, 0>(fc, d.g, s, q); break;
-        case 1: acc = metro_link<S, 1>(fc, d.g, s, q); break;
-        case 2: acc = metro_link<S, 2>(fc, d.g, s, q); break;
-        default: acc = metro_link<S, 3>(fc, d.g, s, q); break;
-        }
+        acc = metro_link_mu<S, CAND, false>(fc, d.g, mu, s, q);
     }
     if (acc) atomicAdd(accepted + c, (unsigned long long)acc);
 }
@@ -516,6 +511,18 @@ int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds)
     return LYM_OK;
 }
 
+extern "C++" template <typename S>
+static void launch_batch_metropolis(lym_batch *b, unsigned nb, int mu, const lym::MetroParams &p)
+{
+    auto *f = static_cast<lym::F2<S> *>(b->W[b->cur]);
+    if (p.candidate == LYM_CAND_SU2)
+        lym::k_batch_metropolis<S, 0><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+    else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max)
+        lym::k_batch_metropolis<S, 1><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+    else  // every candidate has c1 < 8 eps^2: series-only exponential
+        lym::k_batch_metropolis<S, 2><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+}
+
 int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,
                          const uint64_t *seeds, int64_t *accepted)
 {
@@ -538,11 +545,9 @@ int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int ns
             for (int par = 0; par < 2; par++) {
                 p.parity = par;
                 if (b->prec == LYM_PREC_FP64)
-                    lym::k_batch_metropolis<double><<<nb, 256, 0, b->stream>>>(
-                        static_cast<double2 *>(b->W[b->cur]), b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+                    launch_batch_metropolis<double>(b, nb, mu, p);
                 else
-                    lym::k_batch_metropolis<float><<<nb, 256, 0, b->stream>>>(
-                        static_cast<float2 *>(b->W[b->cur]), b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+                    launch_batch_metropolis<float>(b, nb, mu, p);
                 BLAUNCHED(b, "k_batch_metropolis");
             }
     }

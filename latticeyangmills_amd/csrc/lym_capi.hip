@@ -420,6 +420,30 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
     return LYM_OK;
 }
 
+// One Metropolis sub-sweep launch; the candidate kind and the test mode pick
+// the kernel instantiation, so each carries only its own registers.
+extern "C++" template <typename S>
+static void launch_metropolis(lym_ctx *c, unsigned nb, int mu, const lym::MetroParams &p)
+{
+    auto *f = static_cast<lym::F2<S> *>(c->W[c->cur]);
+    if (p.candidate == LYM_CAND_SU2) {
+        if (p.predrawn)
+            lym::k_metropolis<S, 0, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+        else
+            lym::k_metropolis<S, 0, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+    } else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max) {
+        if (p.predrawn)
+            lym::k_metropolis<S, 1, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+        else
+            lym::k_metropolis<S, 1, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+    } else {  // every candidate has c1 < 8 eps^2: series-only exponential
+        if (p.predrawn)
+            lym::k_metropolis<S, 2, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+        else
+            lym::k_metropolis<S, 2, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+    }
+}
+
 int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,
                    uint64_t seed, const double *predrawn, int64_t *accepted)
 {
@@ -454,11 +478,9 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
             for (int par = 0; par < 2; par++) {
                 p.parity = par;
                 if (c->prec == LYM_PREC_FP64)
-                    lym::k_metropolis<double><<<nb, 256, 0, c->stream>>>(static_cast<double2 *>(c->W[c->cur]), c->g,
-                                                                          mu, p, c->accepted);
+                    launch_metropolis<double>(c, nb, mu, p);
                 else
-                    lym::k_metropolis<float><<<nb, 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), c->g, mu,
-                                                                         p, c->accepted);
+                    launch_metropolis<float>(c, nb, mu, p);
                 LYM_LAUNCHED(c, "k_metropolis");
             }
     }

@@ -185,19 +185,6 @@ template <typename R> LYM_DI void mul_na_rows01(su3<R> &r, const su3<R> &a, cons
             r.e[3 * i + j] = s;
         }
 }
-// Rows 0, 1 of a * b (a: rows 0, 1 read; b: all rows)
-template <typename R> LYM_DI void mul_nn_rows01(su3<R> &r, const su3<R> &a, const su3<R> &b)
-{
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            cplx<R> s{R(0), R(0)};
-#pragma unroll
-            for (int k = 0; k < 3; k++) s = cfma(a.e[3 * i + k], b.e[3 * k + j], s);
-            r.e[3 * i + j] = s;
-        }
-}
 // Columns 0, 1 of a * b, i.e. rows 0, 1 of (a * b)^+ after conjugation
 // (a: all rows; b: columns 0, 1 read); stored as r.e[3*i + j] for j < 2.
 template <typename R> LYM_DI void mul_nn_cols01(su3<R> &r, const su3<R> &a, const su3<R> &b)
@@ -449,10 +436,11 @@ LYM_DI su3<double> exp_assemble(const herm8<double> &q, const su3<double> &q2, d
 // Q^3 = c1 Q + c0 1 (c1 = Tr Q^2 / 2, c0 = det Q), so Q^n = a_n + b_n Q + d_n Q^2
 // with a_{n+1} = c0 d_n, b_{n+1} = a_n + c1 d_n, d_{n+1} = b_n, and
 // f_j = sum_n i^n coef_{j,n} / n!.  The spectral radius is at most
-// sqrt(4 c1 / 3); for c1 <= 0.3 the 18 terms kept leave a truncation error below
-// 1e-19, so this agrees with the Morningstar-Peardon form to rounding and
-// needs no transcendental function.
-constexpr double kSeriesC1Max = 0.3;
+// sqrt(4 c1 / 3) <= 0.817 for c1 <= 0.5, so the terms n = 0..18 kept leave a
+// truncation error below 2e-19 (the first omitted term, 0.817^19 / 19!); the
+// result agrees with the Morningstar-Peardon form to rounding (4.6e-16 against
+// a Pade matrix exponential at c1 = 0.5) and needs no transcendental function.
+constexpr double kSeriesC1Max = 0.5;
 constexpr int kSeriesTerms = 18;
 // Short tier: c1 <= 0.004 (spectral radius <= 0.073) needs 9 terms for a
 // truncation error below 2e-18.  The Wilson flow at eps <= 0.02 is almost
@@ -500,6 +488,17 @@ LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, 
     else
         series_coeffs<kSeriesTerms>(c0, c1, f0, f1, f2);
     return exp_assemble(q, q2, d2, f0, f1, f2);
+}
+
+// exp(iQ) by the series alone, for Q known to have c1 = Tr Q^2 / 2 <= kSeriesC1Max
+// (Metropolis candidates exp(i eps H) with 8 eps^2 <= kSeriesC1Max): no
+// transcendental branch, and so far fewer registers than exp_iq.
+LYM_DI su3<double> exp_iq_small(const herm8<double> &q)
+{
+    const double d2 = -q.d0 - q.d1;
+    su3<double> q2 = herm_square(q);
+    const double c1 = 0.5 * (q2.e[0].re + q2.e[4].re + q2.e[8].re);
+    return exp_iq_series(q, q2, d2, c1);
 }
 
 // exp(iQ) for Hermitian traceless Q: Cayley-Hamilton form of Morningstar &

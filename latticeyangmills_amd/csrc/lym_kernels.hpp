@@ -591,7 +591,9 @@ LYM_DI su3<double> cand_su2_product(const double *u, double eps)
 }
 
 // X = exp(i eps H), H = sum_a (2u_a - 1) lambda_a (Gell-Mann matrices).
-LYM_DI su3<double> cand_exph(const double *u, double eps)
+// Tr (eps H)^2 / 2 = eps^2 sum_a h_a^2 < 8 eps^2, so for 8 eps^2 <= kSeriesC1Max
+// (eps <= 0.25) SMALL takes the series-only exponential.
+template <bool SMALL> LYM_DI su3<double> cand_exph(const double *u, double eps)
 {
     double h[8];
 #pragma unroll
@@ -603,7 +605,10 @@ LYM_DI su3<double> cand_exph(const double *u, double eps)
     q.o01 = cplx<double>{eps * h[0], -eps * h[1]};
     q.o02 = cplx<double>{eps * h[3], -eps * h[4]};
     q.o12 = cplx<double>{eps * h[5], -eps * h[6]};
-    return exp_iq(q);
+    if constexpr (SMALL)
+        return exp_iq_small(q);
+    else
+        return exp_iq(q);
 }
 
 struct MetroParams {
@@ -619,9 +624,15 @@ struct MetroParams {
 // dS = -(beta/3) Re Tr[(U' - U) Omega], accepted iff exp(-dS) > u
 // (src/Apps/gaugefieldfactory.cpp:205-220, src/Actions/puregauge.cpp:28).
 // Arithmetic in FP64; FP32 storage is reunitarised before the store.
-template <typename S, int MU>
+// CAND (0: R S T product, 13 uniforms per hit; 1: exp(i eps H), 9; 2: the
+// same with 8 eps^2 <= kSeriesC1Max, series-only exponential) and
+// PREDRAWN (uniforms read from an array instead of drawn from Philox) are
+// compile-time, so each kernel carries the registers of one candidate only.
+template <typename S, int MU, int CAND, bool PREDRAWN>
 LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroParams &p)
 {
+    constexpr int NU = (CAND == 0) ? 13 : 9;
+    constexpr int BLOCKS = (NU + 3) / 4;  // Philox outputs 4 uniforms per call
     const long V = g.V;
     Nbr n = neighbours(g, s);
     su3<double> om = staple<double, S, MU>(f, V, s, n);
@@ -631,15 +642,14 @@ LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroPa
     const uint2 key = make_uint2(uint32_t(p.seed), uint32_t(p.seed >> 32));
     int acc = 0;
     for (int hit = 0; hit < p.nhit; hit++) {
-        double u[16];
-        if (p.predrawn) {
-            const double *src = p.predrawn + ((long)lk * p.nhit + hit) * p.nu;
+        double u[4 * BLOCKS];
+        if constexpr (PREDRAWN) {
+            const double *src = p.predrawn + ((long)lk * p.nhit + hit) * NU;
 #pragma unroll
-            for (int k = 0; k < 13; k++)
-                if (k < p.nu) u[k] = src[k];
+            for (int k = 0; k < NU; k++) u[k] = src[k];
         } else {
 #pragma unroll
-            for (uint32_t b = 0; b < 4; b++) {
+            for (uint32_t b = 0; b < BLOCKS; b++) {
                 uint4 o = philox4x32_10(make_uint4(lk, p.sweep, uint32_t(hit), b), key);
                 u[4 * b] = u32_unit(o.x);
                 u[4 * b + 1] = u32_unit(o.y);
@@ -647,14 +657,17 @@ LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroPa
                 u[4 * b + 3] = u32_unit(o.w);
             }
         }
-        su3<double> X = (p.candidate == 0) ? cand_su2_product(u, p.eps) : cand_exph(u, p.eps);
+        su3<double> X;
+        if constexpr (CAND == 0)
+            X = cand_su2_product(u, p.eps);
+        else
+            X = cand_exph<CAND == 2>(u, p.eps);
         su3<double> Un = mul_nn(X, U);
         su3<double> d;
 #pragma unroll
         for (int k = 0; k < 9; k++) d.e[k] = cplx<double>{Un.e[k].re - U.e[k].re, Un.e[k].im - U.e[k].im};
         double dS = -p.beta / 3.0 * retr_mul(d, om);
-        const double uacc = (p.candidate == 0) ? u[12] : u[8];
-        if (exp(-dS) > uacc) {
+        if (exp(-dS) > u[NU - 1]) {
             U = Un;
             acc++;
         }
@@ -662,6 +675,18 @@ LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroPa
     if constexpr (sizeof(S) == 4) reunitarize(U);
     store_link<S, double>(Fm, V, s, U);
     return acc;
+}
+
+// metro_link for a direction known only at run time (uniform over the wave)
+template <typename S, int CAND, bool PREDRAWN>
+LYM_DI int metro_link_mu(F2<S> *__restrict__ f, const Geom g, int mu, long s, const MetroParams &p)
+{
+    switch (mu) {
+    case 0: return metro_link<S, 0, CAND, PREDRAWN>(f, g, s, p);
+    case 1: return metro_link<S, 1, CAND, PREDRAWN>(f, g, s, p);
+    case 2: return metro_link<S, 2, CAND, PREDRAWN>(f, g, s, p);
+    default: return metro_link<S, 3, CAND, PREDRAWN>(f, g, s, p);
+    }
 }
 
 // site index of the i-th site of the given parity (requires even extents)
@@ -674,21 +699,13 @@ LYM_DI long parity_site(const Geom g, long i, int parity)
     return x + (long)g.L[0] * r;
 }
 
-template <typename S>
-__global__ __launch_bounds__(256) void k_metropolis(F2<S> *__restrict__ f, Geom g, int mu, MetroParams p,
+template <typename S, int CAND, bool PREDRAWN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(F2<S> *__restrict__ f, Geom g, int mu, MetroParams p,
                                                     unsigned long long *__restrict__ accepted)
 {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     int acc = 0;
-    if (i < g.V / 2) {
-        const long s = parity_site(g, i, p.parity);
-        switch (mu) {
-        case 0: acc = metro_link<S, 0>(f, g, s, p); break;
-        case 1: acc = metro_link<S, 1>(f, g, s, p); break;
-        case 2: acc = metro_link<S, 2>(f, g, s, p); break;
-        default: acc = metro_link<S, 3>(f, g, s, p); break;
-        }
-    }
+    if (i < g.V / 2) acc = metro_link_mu<S, CAND, PREDRAWN>(f, g, mu, parity_site(g, i, p.parity), p);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
     if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepted, (unsigned long long)acc);

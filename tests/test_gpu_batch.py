@@ -87,16 +87,17 @@ def test_graph_replays_match_eager(inputs):
                 assert np.array_equal(b.download(c), eager_links[c]), (rep, c)
 
 
-def test_metropolis_matches_single_lattice_path(inputs):
+@pytest.mark.parametrize("eps", [0.24, 0.3])
+def test_metropolis_matches_single_lattice_path(inputs, eps):
     seeds = [11, 22, 33, 44]
     with BatchContext(DIMS, "fp64") as b:
         for c, L in enumerate(inputs):
             b.upload(c, L)
-        acc = b.metropolis(6.0, 0.24, 4, 2, seeds, first_sweep=3)
+        acc = b.metropolis(6.0, eps, 4, 2, seeds, first_sweep=3)
         for c, L in enumerate(inputs):
             with LatticeContext(DIMS[c], "fp64") as ctx:
                 ctx.upload(L)
-                a1 = ctx.metropolis(6.0, 0.24, 4, 2, 3, "exph", seeds[c])
+                a1 = ctx.metropolis(6.0, eps, 4, 2, 3, "exph", seeds[c])
                 assert a1 == acc[c]
                 assert np.array_equal(ctx.download(), b.download(c))
 

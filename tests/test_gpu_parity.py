@@ -134,10 +134,12 @@ def test_flow_measure_every(therm8):
             assert np.all(np.isnan(every[r, 1:]))
 
 
-@pytest.mark.parametrize("candidate", ["su2", "exph"])
+@pytest.mark.parametrize("candidate,eps", [("su2", 0.24), ("exph", 0.24), ("exph", 0.3)])
 @pytest.mark.parametrize("dims", [(4, 4, 4, 4), (2, 4, 6, 4)])
-def test_metropolis_predrawn_per_link(dims, candidate):
-    """Test mode: identical pre-drawn uniforms give identical links."""
+def test_metropolis_predrawn_per_link(dims, candidate, eps):
+    """Test mode: identical pre-drawn uniforms give identical links.  exp(i eps H)
+    candidates take the series-only exponential for 8 eps^2 <= 0.5 (eps 0.24)
+    and the general one above (eps 0.3)."""
     V = orc.volume(dims)
     nhit, nsweeps = 5, 2
     nu = 13 if candidate == "su2" else 9
@@ -145,25 +147,25 @@ def test_metropolis_predrawn_per_link(dims, candidate):
     u = rng.random(nsweeps * 4 * V * nhit * nu)
     links = orc.hot_start(dims, 8)
     cid = 0 if candidate == "su2" else 1
-    want, acc_want = orc.metropolis(links, dims, 6.0, 0.24, nhit, nsweeps, 0, cid, 0, u)
+    want, acc_want = orc.metropolis(links, dims, 6.0, eps, nhit, nsweeps, 0, cid, 0, u)
     with LatticeContext(dims) as ctx:
         ctx.upload(links)
-        acc = ctx.metropolis(6.0, 0.24, nhit, nsweeps, 0, candidate, 0, u)
+        acc = ctx.metropolis(6.0, eps, nhit, nsweeps, 0, candidate, 0, u)
         got = ctx.download()
     assert acc == acc_want
     assert np.max(np.abs(got - want)) < 1e-12
 
 
-@pytest.mark.parametrize("candidate", ["su2", "exph"])
-def test_metropolis_philox_per_link(candidate):
+@pytest.mark.parametrize("candidate,eps", [("su2", 0.24), ("exph", 0.24), ("exph", 0.3)])
+def test_metropolis_philox_per_link(candidate, eps):
     """Production mode: the oracle draws the same Philox stream."""
     dims = (4, 4, 4, 4)
     links = orc.hot_start(dims, 4)
     cid = 0 if candidate == "su2" else 1
-    want, acc_want = orc.metropolis(links, dims, 6.0, 0.24, 10, 3, 7, cid, 2024)
+    want, acc_want = orc.metropolis(links, dims, 6.0, eps, 10, 3, 7, cid, 2024)
     with LatticeContext(dims) as ctx:
         ctx.upload(links)
-        acc = ctx.metropolis(6.0, 0.24, 10, 3, 7, candidate, 2024)
+        acc = ctx.metropolis(6.0, eps, 10, 3, 7, candidate, 2024)
         got = ctx.download()
     assert acc == acc_want
     assert np.max(np.abs(got - want)) < 1e-11

@@ -149,3 +149,59 @@ def test_flow_row_count_follows_reference():
     f.argtypes = [ctypes.c_double, ctypes.c_double]
     assert [f(t, e) for t, e in ((1.0, 0.01), (4.0, 0.01), (6.0, 0.02), (2.0, 0.01), (8.0, 0.02))] == \
         [100, 400, 300, 200, 400]
+
+
+def weak_abelian_mode(dims, alpha, plane_fields):
+    """Diagonal links U_mu = exp(i A_mu h) for smooth abelian fields A_mu(x):
+    plane_fields maps mu -> (h, function of the site coordinates)."""
+    V = orc.volume(dims)
+    s = np.arange(V)
+    coords = [s % dims[0], (s // dims[0]) % dims[1], (s // (dims[0] * dims[1])) % dims[2],
+              s // (dims[0] * dims[1] * dims[2])]
+    U = np.zeros((4, V, 3, 3), complex)
+    U[:] = np.eye(3)
+    for mu, (h, f) in plane_fields.items():
+        a = alpha * f(coords)
+        for k in range(3):
+            U[mu, :, k, k] = np.exp(1j * a * h[k])
+    return orc.to18(U)
+
+
+def test_energy_density_normalisation_on_smooth_field():
+    """a^4 E = -sum Re Tr C^2 / 64 is the continuum E = sum_{mu<nu} tr F^2:
+    for A_0 = a sin(k y) h1 and A_2 = a cos(k t) h2 (tr h^2 = 2) the volume
+    average is 2 a^2 k^2, and the plaquette form 36 (1 - plaq) agrees; both up
+    to O(k^2) lattice corrections, so they converge as L grows."""
+    h1 = np.array([1.0, -1.0, 0.0])
+    h2 = np.array([1.0, 1.0, -2.0]) / np.sqrt(3)
+    prev = None
+    for L in (16, 32):
+        dims = (2, L, 2, L)
+        k = 2 * np.pi / L
+        links = weak_abelian_mode(dims, 0.05, {0: (h1, lambda c: np.sin(k * c[1])),
+                                               2: (h2, lambda c: np.cos(k * c[3]))})
+        plaq, energy, _, _ = orc.observables(links, dims)
+        V = orc.volume(dims)
+        e_clover = -energy / (64 * V)
+        e_cont = 2 * 0.05 ** 2 * k ** 2
+        assert abs(e_clover / (36 * (1 - plaq)) - 1) < 0.04
+        err = abs(e_clover / e_cont - 1)
+        assert err < 0.06
+        if prev is not None:
+            assert err < prev / 3  # O(k^2): halving k quarters the deviation
+        prev = err
+
+
+def test_flow_time_is_heat_equation_time():
+    """A weak abelian mode A_0 = a sin(k y) h1 obeys the heat equation under
+    the flow, dA/dt = Laplacian A, so E(t) = E(0) exp(-2 khat^2 t) with
+    khat^2 = 4 sin^2(k/2).  Pins the 1/2 in Z, the RK3 coefficients and the
+    flow-time scale t (in units of a^2) together."""
+    L = 16
+    dims = (2, L, 2, 2)
+    k = 2 * np.pi / L
+    links = weak_abelian_mode(dims, 0.01, {0: (np.array([1.0, -1.0, 0.0]), lambda c: np.sin(k * c[1]))})
+    series, _ = orc.flow(links, dims, 0.01, 1.01, 25, 0)
+    kh2 = 4 * np.sin(k / 2) ** 2
+    for t, _, _, e in series[::25]:
+        assert abs(e / series[0, 3] - np.exp(-2 * kh2 * t)) < 1e-5, t

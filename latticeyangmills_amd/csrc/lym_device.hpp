@@ -369,14 +369,18 @@ struct Nbr {
     int fwd[4], bwd[4];
 };
 
+// Site indices fit in 32 bits (V < 2^31 / (9 * sizeof(complex)), checked when
+// a lattice is created), so the coordinates come from unsigned 32-bit
+// division rather than the far longer 64-bit sequence.
 LYM_DI Nbr neighbours(const Geom g, long s)
 {
     Nbr n;
-    long r = s;
+    uint32_t r = uint32_t(s);
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-        int c = (d < 3) ? int(r % g.L[d]) : int(r);
-        if (d < 3) r /= g.L[d];
+        const uint32_t L = uint32_t(g.L[d]);
+        int c = (d < 3) ? int(r % L) : int(r);
+        if (d < 3) r /= L;
         n.fwd[d] = (c == g.L[d] - 1) ? -(g.L[d] - 1) * g.stride[d] : g.stride[d];
         n.bwd[d] = (c == 0) ? (g.L[d] - 1) * g.stride[d] : -g.stride[d];
     }
@@ -676,8 +680,10 @@ LYM_DI uint4 philox4x32_10(uint4 c, uint2 k)
 {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        // one 32x32 -> 64-bit product gives both halves (v_mad_u64_u32)
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c.x, p1 = uint64_t(0xCD9E8D57u) * c.z;
+        const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+        const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
         c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
         k.x += 0x9E3779B9u;
         k.y += 0xBB67AE85u;

@@ -200,7 +200,7 @@ template <typename S, int STAGE, bool REUNIT, int LDS>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
                        const Geom g, long site0, double eps)
 {
-    const int mu = threadIdx.x >> 6;
+    const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
     const int lane = threadIdx.x & 63;
     const long s = site0 + lane;
     const bool active = s < g.V;
@@ -689,14 +689,16 @@ LYM_DI int metro_link_mu(F2<S> *__restrict__ f, const Geom g, int mu, long s, co
     }
 }
 
-// site index of the i-th site of the given parity (requires even extents)
+// site index of the i-th site of the given parity (requires even extents);
+// 32-bit arithmetic, as site indices fit in 32 bits
 LYM_DI long parity_site(const Geom g, long i, int parity)
 {
-    const long half = g.L[0] / 2;
-    const long r = i / half, xh = i % half;
-    const long y = r % g.L[1], z = (r / g.L[1]) % g.L[2], t = r / ((long)g.L[1] * g.L[2]);
-    const long x = 2 * xh + ((parity + y + z + t) & 1);
-    return x + (long)g.L[0] * r;
+    const uint32_t half = uint32_t(g.L[0]) / 2u, L1 = uint32_t(g.L[1]), L2 = uint32_t(g.L[2]);
+    const uint32_t r = uint32_t(i) / half, xh = uint32_t(i) - r * half;
+    const uint32_t r1 = r / L1, y = r - r1 * L1;
+    const uint32_t t = r1 / L2, z = r1 - t * L2;
+    const uint32_t x = 2u * xh + ((uint32_t(parity) + y + z + t) & 1u);
+    return long(x + uint32_t(g.L[0]) * r);
 }
 
 template <typename S, int CAND, bool PREDRAWN>

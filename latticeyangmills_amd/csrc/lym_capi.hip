@@ -425,22 +425,23 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
 extern "C++" template <typename S>
 static void launch_metropolis(lym_ctx *c, unsigned nb, int mu, const lym::MetroParams &p)
 {
+    const unsigned per_xcd = (nb + 7) / 8;
     auto *f = static_cast<lym::F2<S> *>(c->W[c->cur]);
     if (p.candidate == LYM_CAND_SU2) {
         if (p.predrawn)
-            lym::k_metropolis<S, 0, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 0, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 0, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 0, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
     } else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max) {
         if (p.predrawn)
-            lym::k_metropolis<S, 1, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 1, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 1, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 1, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
     } else {  // every candidate has c1 < 8 eps^2: series-only exponential
         if (p.predrawn)
-            lym::k_metropolis<S, 2, true><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 2, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 2, false><<<nb, 256, 0, c->stream>>>(f, c->g, mu, p, c->accepted);
+            lym::k_metropolis<S, 2, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
     }
 }
 

@@ -701,11 +701,16 @@ LYM_DI long parity_site(const Geom g, long i, int parity)
     return long(x + uint32_t(g.L[0]) * r);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; workgroup b works on
+// block (b % 8) * per_xcd + b / 8, so each XCD sweeps one contiguous range
+// of the lattice and the neighbour links along y and z stay in its L2.  The
+// grid is 8 * per_xcd workgroups; blocks past the lattice return at once.
 template <typename S, int CAND, bool PREDRAWN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(F2<S> *__restrict__ f, Geom g, int mu, MetroParams p,
-                                                    unsigned long long *__restrict__ accepted)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(
+    F2<S> *__restrict__ f, Geom g, int mu, MetroParams p, unsigned per_xcd, unsigned long long *__restrict__ accepted)
 {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const unsigned blk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const long i = (long)blk * 256 + threadIdx.x;
     int acc = 0;
     if (i < g.V / 2) acc = metro_link_mu<S, CAND, PREDRAWN>(f, g, mu, parity_site(g, i, p.parity), p);
 #pragma unroll

@@ -98,6 +98,13 @@ int lym_flow(lym_ctx *ctx, double eps, double tau, int measure_every, double *se
 int lym_metropolis(lym_ctx *ctx, double beta, double eps, int nhit, int nsweeps, int first_sweep,
                    int candidate, uint64_t seed, const double *predrawn, int64_t *accepted);
 
+/* Health of the stored links, reported and never fatal.  The reference
+ * (src/Math/su3.h, no checks anywhere) lets NaN propagate silently.
+ * *max_dev = largest over all links of max_ij |(U U^+ - 1)_ij| and |det U - 1|
+ * (finite entries only); *nonfinite = number of links with a NaN or infinite
+ * entry.  Either pointer may be NULL. */
+int lym_check_links(lym_ctx *ctx, double *max_dev, int64_t *nonfinite);
+
 /* Blocks until all work on the context's stream is done. */
 int lym_synchronize(lym_ctx *ctx);
 
@@ -114,8 +121,10 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
  * (workgroup b takes 64-site run b), 1 = LYM_SCHED_CONTIG (each of the 8 XCDs
  * takes a contiguous eighth), 2 = LYM_SCHED_COLUMN (each XCD marches columns of
  * about `target` sites of the (x, y, z) volume along t, so neighbouring time
- * slices stay in its L2).  Results do not depend on the schedule.  Default:
- * COLUMN | LDS with 8192 sites (fastest on 24^4 .. 48^3x96 in both precisions). */
+ * slices stay in its L2).  Results agree across schedules to rounding: the
+ * exponential's series length is chosen per wavefront, and a schedule decides
+ * which sites share one.  Default: COLUMN | LDS with 8192 sites; the fastest
+ * choice depends on shape and precision, which lym_autotune measures. */
 #define LYM_SCHED_LINEAR 0
 #define LYM_SCHED_CONTIG 1
 #define LYM_SCHED_COLUMN 2

@@ -50,6 +50,7 @@ def _sig(lib):
     lib.lym_metropolis.argtypes = [_P, c.c_double, c.c_double, _I, _I, _I, _I, c.c_uint64, _D,
                                    c.POINTER(c.c_int64)]
     lib.lym_synchronize.argtypes = [_P]
+    lib.lym_check_links.argtypes = [_P, c.POINTER(c.c_double), c.POINTER(c.c_int64)]
     lib.lym_profile_flow.argtypes = [_P, c.c_double, _I, _I, _D, _D]
     lib.lym_copy_bandwidth.argtypes = [_I, c.c_int64, _I, _D]
     lib.lym_set_schedule.argtypes = [_P, _I, _I]
@@ -169,6 +170,12 @@ class LatticeContext:
 
     def synchronize(self):
         self._check(self._lib.lym_synchronize(self._h), "lym_synchronize")
+
+    def check_links(self):
+        """(max unitarity / determinant deviation, links with non-finite entries)"""
+        dev, bad = ctypes.c_double(0), ctypes.c_int64(0)
+        self._check(self._lib.lym_check_links(self._h, ctypes.byref(dev), ctypes.byref(bad)), "lym_check_links")
+        return dev.value, bad.value
 
     def profile_flow(self, eps, nsteps, measure_every=0):
         total = ctypes.c_double(0)

@@ -32,6 +32,7 @@ struct lym_ctx {
     int obs_cap = 0;
     double *staging = nullptr;  // V x 18 doubles (reference layout of one direction)
     unsigned long long *accepted = nullptr;
+    unsigned long long *check = nullptr;  // lym_check_links: max deviation bits, non-finite links
     double *predrawn = nullptr;
     size_t predrawn_cap = 0;
     std::vector<hipEvent_t> events;
@@ -231,6 +232,7 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
     if (hipMalloc(&c->partials, sizeof(ksum) * 4 * size_t(c->nblk_obs)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->staging, sizeof(double) * 18 * size_t(V)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&c->check, 2 * sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
     if (apply_schedule(c, lym::SCHED_COLUMN | lym::SCHED_LDS, 8192) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
@@ -246,7 +248,7 @@ int lym_destroy(lym_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
-                    (void *)c->accepted, (void *)c->predrawn, (void *)c->d_order})
+                    (void *)c->accepted, (void *)c->check, (void *)c->predrawn, (void *)c->d_order})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -489,6 +491,28 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     LYM_HIP(c, hipMemcpyAsync(&acc, c->accepted, sizeof acc, hipMemcpyDeviceToHost, c->stream));
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     if (accepted) *accepted = int64_t(acc);
+    return LYM_OK;
+}
+
+int lym_check_links(lym_ctx *c, double *max_dev, int64_t *nonfinite)
+{
+    if (!c) return LYM_ERR_ARG;
+    unsigned long long *d = c->check;
+    lym::k_zero_u64<<<1, 64, 0, c->stream>>>(d, 2);
+    LYM_LAUNCHED(c, "k_zero_u64");
+    const unsigned nb = blocks_for(4 * c->g.V, 256);
+    if (c->prec == LYM_PREC_FP64)
+        lym::k_check_links<double><<<nb, 256, 0, c->stream>>>(static_cast<double2 *>(c->W[c->cur]), c->g.V, d);
+    else
+        lym::k_check_links<float><<<nb, 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), c->g.V, d);
+    LYM_LAUNCHED(c, "k_check_links");
+    unsigned long long h[2];
+    LYM_HIP(c, hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    double v;
+    std::memcpy(&v, &h[0], sizeof v);
+    if (max_dev) *max_dev = v;
+    if (nonfinite) *nonfinite = int64_t(h[1]);
     return LYM_OK;
 }
 

@@ -414,6 +414,51 @@ static __global__ void k_zero_u64(unsigned long long *p, int n)
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0ull;
 }
 
+// Link health: out[0] = max over links of max_ij |(U U^+ - 1)_ij| and
+// |det U - 1| as the bit pattern of a non-negative double (which orders like
+// its value, so an unsigned 64-bit max is a max of the values); out[1] =
+// links with a non-finite entry.  out[] must be zero before the launch.
+template <typename S>
+static __global__ __launch_bounds__(256) void k_check_links(const F2<S> *__restrict__ f, long V,
+                                                           unsigned long long *__restrict__ out)
+{
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;  // link mu*V + s (4V < 2^31)
+    double dev = 0.0;
+    bool bad = false;
+    if (l < uint32_t(4 * V)) {
+        const uint32_t mu = l / uint32_t(V), s = l - mu * uint32_t(V);
+        const su3<double> u = load_link<double, S>(f + long(mu) * 9 * V, V, s);
+#pragma unroll
+        for (int k = 0; k < 9; k++) bad = bad || !isfinite(u.e[k].re) || !isfinite(u.e[k].im);
+        const su3<double> p = mul_na(u, u);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const double re = p.e[3 * i + j].re - (i == j ? 1.0 : 0.0), im = p.e[3 * i + j].im;
+                dev = fmax(dev, sqrt(re * re + im * im));
+            }
+        // det U = u0 . (u1 x u2) over rows
+        cplx<double> det{0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int k = (j + 1) % 3, m = (j + 2) % 3;
+            cplx<double> c = cfma(u.e[3 + k], u.e[6 + m], cplx<double>{0.0, 0.0});
+            c = cfma(cplx<double>{-u.e[3 + m].re, -u.e[3 + m].im}, u.e[6 + k], c);
+            det = cfma(u.e[j], c, det);
+        }
+        dev = fmax(dev, sqrt((det.re - 1.0) * (det.re - 1.0) + det.im * det.im));
+        if (bad) dev = 0.0;
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) dev = fmax(dev, __shfl_xor(dev, m, 64));
+    const unsigned long long nbad = __popcll(__ballot(bad));
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, (unsigned long long)__double_as_longlong(dev));
+        if (nbad) atomicAdd(out + 1, nbad);
+    }
+}
+
 // reference layout (V x 18 doubles for one mu) -> SoA plane of direction mu
 template <typename S>
 __global__ void k_aos_to_soa(const double *__restrict__ aos, F2<S> *__restrict__ plane, long V)

@@ -43,6 +43,15 @@ public:
     void toDevice();  // upload if the host copy is newer
     void toHost();    // download if the device copy is newer
 
+    // Link health, reported and never fatal (lym_check_links): the largest
+    // deviation of any link from SU(3) and the number of links holding a NaN
+    // or an infinity.  Uploads first if the host copy is newer.
+    struct Health {
+        double maxDeviation;
+        int64_t nonFinite;
+    };
+    Health health();
+
     // throws std::runtime_error with lym_last_error() on failure
     void check(int rc, const char *what);
 

@@ -108,6 +108,13 @@ std::vector<std::array<double, 4>> GpuWilsonFlow::applyWilsonFlow(int confNum)
             "lym_flow");
     d.deviceModified();
     series.resize(size_t(got));
+    // Report a field that left SU(3) or holds NaN (the reference would carry
+    // on silently); the run continues.
+    const DeviceField::Health h = d.health();
+    const double allowed = m_precision == LYM_PREC_FP64 ? 1e-10 : 1e-5;
+    if (h.nonFinite > 0 || h.maxDeviation > allowed)
+        fprintf(stderr, "warning: conf %d after the flow: %lld links non-finite, max deviation from SU(3) %.3g\n",
+                confNum, (long long)h.nonFinite, h.maxDeviation);
     if (!series.empty()) {
         // the observables hold the values at the last measured row
         for (auto *o : m_obs) {

@@ -123,3 +123,11 @@ void DeviceField::toHost()
     check(lym_download_links(m_ctx, p), "lym_download_links");
     m_deviceNewer = false;
 }
+
+DeviceField::Health DeviceField::health()
+{
+    toDevice();
+    Health h{0.0, 0};
+    check(lym_check_links(m_ctx, &h.maxDeviation, &h.nonFinite), "lym_check_links");
+    return h;
+}

@@ -58,6 +58,8 @@ def test_cfg2_first_rows_against_oracle():
         series_g = ctx.flow(0.01, 0.03, 1)  # rows t = 0, 0.01, 0.02
     series_o, _ = orc.flow(links, dims, 0.01, 0.03, 1, 0)
     scale = orc.observables(links, dims)[3]
+    # statistical pin: the beta = 6.0 Wilson plaquette is 0.5937 in equilibrium
+    assert abs(series_o[0, 1] - 0.5937) < 0.002
     for r in range(3):
         assert rel(series_g[r, 1], series_o[r, 1]) < 1e-12, r
         assert rel(series_g[r, 3], series_o[r, 3]) < 1e-12, r

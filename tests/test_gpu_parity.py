@@ -92,13 +92,15 @@ def test_staples_and_derivative(dims, mu):
     assert np.max(np.abs(z - z_ref)) < 1e-12 * np.max(np.abs(z_ref))
 
 
-@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 8, 2)])
-def test_one_flow_step(dims):
+@pytest.mark.parametrize("dims,eps", [((4, 4, 4, 4), 0.02), ((4, 6, 8, 2), 0.02), ((4, 4, 4, 4), 0.5)])
+def test_one_flow_step(dims, eps):
+    """eps = 0.5 on a hot start puts c1 = Tr Q^2 / 2 above 0.5 for most links, so
+    the exponential takes its Morningstar-Peardon branch rather than the series."""
     links = orc.hot_start(dims, 23)
-    want = orc.flow_step(links, dims, 0.02)
+    want = orc.flow_step(links, dims, eps)
     with LatticeContext(dims) as ctx:
         ctx.upload(links)
-        ctx.flow_steps(0.02, 1)
+        ctx.flow_steps(eps, 1)
         got = ctx.download()
     assert np.max(np.abs(got - want)) < 1e-12
 
@@ -227,3 +229,23 @@ def test_lds_variant_without_whole_rows(dims):
         ctx.upload(links)
         ctx.flow_steps(0.02, 1)
         assert np.max(np.abs(ctx.download() - want)) < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
+def test_check_links_reports_health(precision, tol):
+    """NaN and non-unitary links are reported (count, max deviation), not fatal."""
+    dims = (4, 4, 4, 4)
+    links = orc.hot_start(dims, 21)
+    with LatticeContext(dims, precision) as ctx:
+        ctx.upload(links)
+        dev, bad = ctx.check_links()
+        assert dev < tol and bad == 0
+        broken = links.copy()
+        broken[2, 17, 5] = np.nan      # one link with a NaN
+        broken[1, 3, :] *= 1.1         # one link scaled: U U^+ = 1.21, det = 1.331
+        ctx.upload(broken)
+        dev, bad = ctx.check_links()
+        assert bad == 1
+        assert abs(dev - 0.331) < 1e-5
+        ctx.set_cold()
+        assert ctx.check_links() == (0.0, 0)

@@ -185,6 +185,19 @@ template <typename R> LYM_DI void mul_na_rows01(su3<R> &r, const su3<R> &a, cons
             r.e[3 * i + j] = s;
         }
 }
+// Rows 0, 1 of a * b (a: rows 0, 1 read; b: all rows)
+template <typename R> LYM_DI void mul_nn_rows01(su3<R> &r, const su3<R> &a, const su3<R> &b)
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            cplx<R> s{R(0), R(0)};
+#pragma unroll
+            for (int k = 0; k < 3; k++) s = cfma(a.e[3 * i + k], b.e[3 * k + j], s);
+            r.e[3 * i + j] = s;
+        }
+}
 // Columns 0, 1 of a * b, i.e. rows 0, 1 of (a * b)^+ after conjugation
 // (a: all rows; b: columns 0, 1 read); stored as r.e[3*i + j] for j < 2.
 template <typename R> LYM_DI void mul_nn_cols01(su3<R> &r, const su3<R> &a, const su3<R> &b)
@@ -211,6 +224,24 @@ template <typename R> LYM_DI void mul_an_rows01(su3<R> &r, const su3<R> &a, cons
             for (int k = 0; k < 3; k++) s = cfma_cn(a.e[3 * k + i], b.e[3 * k + j], s);
             r.e[3 * i + j] = s;
         }
+}
+// Row 2 of an SU(3) matrix from its rows 0, 1: row2 = conj(row0 x row1).
+template <typename R> LYM_DI void complete_row2(su3<R> &m)
+{
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        const int k = (j + 1) % 3, l = (j + 2) % 3;
+        const cplx<R> a = m.e[k], b = m.e[3 + l], c = m.e[l], d = m.e[3 + k];
+        R re = a.re * b.re;
+        re = fma(-a.im, b.im, re);
+        re = fma(-c.re, d.re, re);
+        re = fma(c.im, d.im, re);
+        R im = a.re * b.im;
+        im = fma(a.im, b.re, im);
+        im = fma(-c.re, d.im, im);
+        im = fma(-c.im, d.re, im);
+        m.e[6 + j] = cplx<R>{re, -im};
+    }
 }
 // acc += m for m in SU(3) given by rows 0, 1: row 2 = conj(row0 x row1).
 template <typename R> LYM_DI void acc_su3_rows01(su3<R> &acc, const su3<R> &m)

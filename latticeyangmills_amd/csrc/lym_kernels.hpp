@@ -253,42 +253,82 @@ __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ wi
 // Clover of plane (MU, NU) at site s (src/Observables/superobs.cpp:40-106):
 // clover = P_{mu nu}(x) + [(-mu,-nu) leaf] - [(+mu,-nu) leaf] - [(-mu,+nu) leaf]
 // with each leaf formed as in the reference; returns C = [clover - clover^+]
-// made traceless (8 reals), and Re Tr P_{mu nu}(x) in *plaq.
+// made traceless (8 reals), and Re Tr P_{mu nu}(x) in *plaq.  In FP32 (mixed
+// path) each leaf, a product of four SU(3) links, is formed from its first two
+// rows and completed by unitarity (240 instead of 324 operations per leaf);
+// FP64 multiplies out all three rows as the reference does.
 template <typename R, typename S, int MU, int NU>
 LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, R *plaq)
 {
     const F2<S> *Fm = f + (long)MU * 9 * V;
     const F2<S> *Fn = f + (long)NU * 9 * V;
+    constexpr bool TWO_ROW = sizeof(R) == 4;
+    auto add = [](su3<R> &cl, const su3<R> &t, R sign) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{fma(sign, t.e[k].re, cl.e[k].re), fma(sign, t.e[k].im, cl.e[k].im)};
+    };
     su3<R> cl;
     {  // P = ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+
-        su3<R> t = mul_nn(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
-        t = mul_na(t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
-        cl = mul_na(t, load_link<R, S>(Fn, V, s));
+        if constexpr (TWO_ROW) {
+            su3<R> t, u;
+            mul_nn_rows01(t, load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
+            mul_na_rows01(u, t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
+            mul_na_rows01(cl, u, load_link<R, S>(Fn, V, s));
+            complete_row2(cl);
+        } else {
+            su3<R> t = mul_nn(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
+            t = mul_na(t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
+            cl = mul_na(t, load_link<R, S>(Fn, V, s));
+        }
         *plaq = retrace(cl);
     }
     {  // y = x-mu-nu: ((U_mu(y+nu)^+ U_nu(y)^+) U_mu(y)) U_nu(y+mu) = (U_nu(y) U_mu(x-mu))^+ U_mu(y) U_nu(x-nu)
         const long y = s + n.bwd[MU] + n.bwd[NU];
-        su3<R> t = mul_nn(load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
-        t = mul_an(t, load_link<R, S>(Fm, V, y));
-        t = mul_nn(t, load_link<R, S>(Fn, V, s + n.bwd[NU]));
-#pragma unroll
-        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re + t.e[k].re, cl.e[k].im + t.e[k].im};
+        su3<R> t;
+        if constexpr (TWO_ROW) {
+            su3<R> p, u;
+            mul_nn_cols01(p, load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
+            mul_an_rows01(u, p, load_link<R, S>(Fm, V, y));
+            mul_nn_rows01(t, u, load_link<R, S>(Fn, V, s + n.bwd[NU]));
+            complete_row2(t);
+        } else {
+            t = mul_nn(load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
+            t = mul_an(t, load_link<R, S>(Fm, V, y));
+            t = mul_nn(t, load_link<R, S>(Fn, V, s + n.bwd[NU]));
+        }
+        add(cl, t, R(1));
     }
     {  // y = x-nu: ((U_mu(x) U_nu(x+mu-nu)^+) U_mu(y)^+) U_nu(y)
         const long y = s + n.bwd[NU];
-        su3<R> t = mul_na(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
-        t = mul_na(t, load_link<R, S>(Fm, V, y));
-        t = mul_nn(t, load_link<R, S>(Fn, V, y));
-#pragma unroll
-        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re - t.e[k].re, cl.e[k].im - t.e[k].im};
+        su3<R> t;
+        if constexpr (TWO_ROW) {
+            su3<R> p, u;
+            mul_na_rows01(p, load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
+            mul_na_rows01(u, p, load_link<R, S>(Fm, V, y));
+            mul_nn_rows01(t, u, load_link<R, S>(Fn, V, y));
+            complete_row2(t);
+        } else {
+            t = mul_na(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
+            t = mul_na(t, load_link<R, S>(Fm, V, y));
+            t = mul_nn(t, load_link<R, S>(Fn, V, y));
+        }
+        add(cl, t, R(-1));
     }
     {  // y = x-mu: ((U_mu(y)^+ U_nu(y)) U_mu(x-mu+nu)) U_nu(x)^+
         const long y = s + n.bwd[MU];
-        su3<R> t = mul_an(load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
-        t = mul_nn(t, load_link<R, S>(Fm, V, y + n.fwd[NU]));
-        t = mul_na(t, load_link<R, S>(Fn, V, s));
-#pragma unroll
-        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{cl.e[k].re - t.e[k].re, cl.e[k].im - t.e[k].im};
+        su3<R> t;
+        if constexpr (TWO_ROW) {
+            su3<R> p, u;
+            mul_an_rows01(p, load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
+            mul_nn_rows01(u, p, load_link<R, S>(Fm, V, y + n.fwd[NU]));
+            mul_na_rows01(t, u, load_link<R, S>(Fn, V, s));
+            complete_row2(t);
+        } else {
+            t = mul_an(load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
+            t = mul_nn(t, load_link<R, S>(Fm, V, y + n.fwd[NU]));
+            t = mul_na(t, load_link<R, S>(Fn, V, s));
+        }
+        add(cl, t, R(-1));
     }
     // C = clover - clover^+ (anti-Hermitian), made traceless
     ah8<R> C;
@@ -314,27 +354,6 @@ template <typename R> LYM_DI R retr_ah(const ah8<R> &A, const ah8<R> &B)
     return -d - R(2) * o;
 }
 
-// Contributions of one plane pair at one site (src/Observables/superobs.cpp:40-124):
-// PAIR 0 = planes (01, 23), 1 = (02, 13), 2 = (03, 12).  out[0] = Re Tr P over
-// both planes, out[1] = Re Tr(C C) over both planes (the reference's raw
-// energy terms, superobs.cpp:109), out[2] = this pair's term of
-// q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124).
-template <typename R, typename S, int PAIR>
-LYM_DI void pair_observables(const F2<S> *__restrict__ f, const Geom g, long s, double out[3])
-{
-    constexpr int M1 = 0, N1 = PAIR + 1;
-    constexpr int M2 = (PAIR == 2) ? 1 : (PAIR == 0 ? 2 : 1), N2 = (PAIR == 0) ? 3 : (PAIR == 1 ? 3 : 2);
-    const long V = g.V;
-    Nbr n = neighbours(g, s);
-    R p1, p2;
-    ah8<R> A = clover<R, S, M1, N1>(f, V, s, n, &p1);
-    ah8<R> B = clover<R, S, M2, N2>(f, V, s, n, &p2);
-    out[0] = double(p1) + double(p2);
-    out[1] = double(retr_ah(A, A)) + double(retr_ah(B, B));
-    const double t = double(retr_ah(A, B));
-    out[2] = (PAIR == 1) ? t : -t;  // -Re Tr(C02 C13^+) = +Re Tr(C02 C13)
-}
-
 // Block reduction of 4 compensated sums; thread 0 returns the block total.
 template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
 {
@@ -356,33 +375,70 @@ template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
     }
 }
 
-// 192-thread workgroups: 64 consecutive sites x 3 plane pairs (wave = pair).
-// Per-block compensated partials of {sum Re Tr P, sum Re Tr C^2, sum q,
-// sum |q terms|}.
+// Observables of one 64-site run with 384 threads (src/Observables/superobs.cpp:31-128).
+// Wave w = 0..5 forms the clover of plane w in the reference's loop order,
+// (01, 02, 03, 12, 13, 23), with Re Tr P and Re Tr C^2 at each site, and
+// leaves C in LDS; after a barrier, wave p = 0..2 forms the term of
+// q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124)
+// of plane pair (p, 5 - p).  One clover per thread keeps the registers of
+// one clover live, not two.  v[] = {Re Tr P, Re Tr C^2, q term, |q term|}.
 constexpr int kObsSites = 64;
+constexpr int kObsThreads = 6 * kObsSites;
 
-template <typename S>
-__global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
-                                                     ksum *__restrict__ partials)
+template <typename R, typename S>
+LYM_DI void observables_block(const F2<S> *__restrict__ f, const Geom g, long s, bool active, ksum v[4])
 {
-    const long run = sched_run(sc, blockIdx.x);
-    const long s = run * kObsSites + (threadIdx.x & 63);
-    const int pair = threadIdx.x >> 6;
-    ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    if (run >= 0 && s < g.V) {
-        double o[3];
-        if (pair == 0)
-            pair_observables<S, S, 0>(f, g, s, o);
-        else if (pair == 1)
-            pair_observables<S, S, 1>(f, g, s, o);
-        else
-            pair_observables<S, S, 2>(f, g, s, o);
-        v[0].s = o[0];
-        v[1].s = o[1];
-        v[2].s = o[2];
-        v[3].s = fabs(o[2]);
+    __shared__ R lds[6 * 9 * kObsSites];
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
+    R *mine = lds + w * 9 * kObsSites + (threadIdx.x & 63);
+    if (active) {
+        const Nbr n = neighbours(g, s);
+        R plaq;
+        ah8<R> C;
+        switch (w) {
+        case 0: C = clover<R, S, 0, 1>(f, g.V, s, n, &plaq); break;
+        case 1: C = clover<R, S, 0, 2>(f, g.V, s, n, &plaq); break;
+        case 2: C = clover<R, S, 0, 3>(f, g.V, s, n, &plaq); break;
+        case 3: C = clover<R, S, 1, 2>(f, g.V, s, n, &plaq); break;
+        case 4: C = clover<R, S, 1, 3>(f, g.V, s, n, &plaq); break;
+        default: C = clover<R, S, 2, 3>(f, g.V, s, n, &plaq); break;
+        }
+        v[0].s = double(plaq);
+        v[1].s = double(retr_ah(C, C));
+        const R c9[9] = {C.a0, C.a1, C.a2, C.o01.re, C.o01.im, C.o02.re, C.o02.im, C.o12.re, C.o12.im};
+#pragma unroll
+        for (int k = 0; k < 9; k++) mine[k * kObsSites] = c9[k];
     }
-    block_ksum4<192>(v, partials + 4 * blockIdx.x);
+    __syncthreads();
+    if (active && w < 3) {
+        auto read = [&](int plane) {
+            const R *o = lds + plane * 9 * kObsSites + (threadIdx.x & 63);
+            ah8<R> C;
+            C.a0 = o[0];
+            C.a1 = o[kObsSites];
+            C.a2 = o[2 * kObsSites];
+            C.o01 = cplx<R>{o[3 * kObsSites], o[4 * kObsSites]};
+            C.o02 = cplx<R>{o[5 * kObsSites], o[6 * kObsSites]};
+            C.o12 = cplx<R>{o[7 * kObsSites], o[8 * kObsSites]};
+            return C;
+        };
+        const double t = double(retr_ah(read(w), read(5 - w)));
+        const double q = (w == 1) ? t : -t;  // -Re Tr(C02 C13^+) = +Re Tr(C02 C13)
+        v[2].s = q;
+        v[3].s = fabs(q);
+    }
+}
+
+// One partial {sum Re Tr P, sum Re Tr C^2, sum q, sum |q terms|} per 64-site run.
+template <typename S>
+__global__ __launch_bounds__(kObsThreads) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
+                                                             ksum *__restrict__ partials)
+{
+    const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
+    const long s = run * kObsSites + (threadIdx.x & 63);
+    ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    observables_block<S, S>(f, g, s, run >= 0 && s < g.V, v);
+    block_ksum4<kObsThreads>(v, partials + 4 * blockIdx.x);
 }
 
 // Sum nblk block partials into dst[0..3] (one workgroup of 1024 threads).

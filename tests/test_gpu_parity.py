@@ -92,10 +92,11 @@ def test_staples_and_derivative(dims, mu):
     assert np.max(np.abs(z - z_ref)) < 1e-12 * np.max(np.abs(z_ref))
 
 
-@pytest.mark.parametrize("dims,eps", [((4, 4, 4, 4), 0.02), ((4, 6, 8, 2), 0.02), ((4, 4, 4, 4), 0.5)])
+@pytest.mark.parametrize("dims,eps", [((4, 4, 4, 4), 0.02), ((4, 6, 8, 2), 0.02), ((4, 4, 4, 4), 2.0)])
 def test_one_flow_step(dims, eps):
-    """eps = 0.5 on a hot start puts c1 = Tr Q^2 / 2 above 0.5 for most links, so
-    the exponential takes its Morningstar-Peardon branch rather than the series."""
+    """On this hot start c1 = Tr Z^2 / 2 has median 3.8, so with eps = 2 even
+    stage 1 (Q = eps Z / 4, median c1 0.95) puts most links above the series
+    bound c1 = 0.5: the exponential takes its Morningstar-Peardon branch."""
     links = orc.hot_start(dims, 23)
     want = orc.flow_step(links, dims, eps)
     with LatticeContext(dims) as ctx:

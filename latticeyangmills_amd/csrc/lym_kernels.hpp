@@ -263,9 +263,10 @@ LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, 
     const F2<S> *Fm = f + (long)MU * 9 * V;
     const F2<S> *Fn = f + (long)NU * 9 * V;
     constexpr bool TWO_ROW = sizeof(R) == 4;
-    auto add = [](su3<R> &cl, const su3<R> &t, R sign) {
+    auto add = [](su3<R> &cl, const su3<R> &t, R sign) {  // sign = +-1: one rounding, as cl +- t
 #pragma unroll
-        for (int k = 0; k < 9; k++) cl.e[k] = cplx<R>{fma(sign, t.e[k].re, cl.e[k].re), fma(sign, t.e[k].im, cl.e[k].im)};
+        for (int k = 0; k < 9; k++)
+            cl.e[k] = cplx<R>{cl.e[k].re + sign * t.e[k].re, cl.e[k].im + sign * t.e[k].im};
     };
     su3<R> cl;
     {  // P = ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+

@@ -72,20 +72,33 @@ __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restric
     }
 }
 
-// observables: one partial per tile (kObsThreads threads: 64 sites x 6 planes)
+// observables: one partial per tile (192 threads = 64 sites x 3 plane pairs)
 template <typename S>
-__global__ __launch_bounds__(kObsThreads) void k_batch_observables(const F2<S> *__restrict__ f,
-                                                                   const ConfDesc *__restrict__ desc, int nconf,
-                                                                   long ntiles, ksum *__restrict__ partials)
+__global__ __launch_bounds__(192) void k_batch_observables(const F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
+                                                           int nconf, long ntiles, ksum *__restrict__ partials)
 {
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int c = find_conf(desc, nconf, tile);
         const ConfDesc d = desc[c];
         const long s = (tile - d.tile0) * kObsSites + (threadIdx.x & 63);
+        const int pair = threadIdx.x >> 6;
         ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-        observables_block<S, S>(f + d.link_off, d.g, s, s < d.g.V, v);
-        block_ksum4<kObsThreads>(v, partials + 4 * tile);
-        __syncthreads();  // LDS of this tile is free for the next
+        if (s < d.g.V) {
+            double o[3];
+            const F2<S> *fc = f + d.link_off;
+            if (pair == 0)
+                pair_observables<S, S, 0>(fc, d.g, s, o);
+            else if (pair == 1)
+                pair_observables<S, S, 1>(fc, d.g, s, o);
+            else
+                pair_observables<S, S, 2>(fc, d.g, s, o);
+            v[0].s = o[0];
+            v[1].s = o[1];
+            v[2].s = o[2];
+            v[3].s = fabs(o[2]);
+        }
+        block_ksum4<192>(v, partials + 4 * tile);
+        __syncthreads();
     }
 }
 
@@ -263,7 +276,7 @@ int step(lym_batch *b, double eps)
 template <typename S> int enqueue_obs(lym_batch *b, int slot)
 {
     const unsigned g = unsigned(std::min<long>(b->ntiles, 65535L * 4));
-    lym::k_batch_observables<S><<<g, lym::kObsThreads, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]), b->d_desc,
+    lym::k_batch_observables<S><<<g, 192, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]), b->d_desc,
                                                           b->nconf, b->ntiles, b->partials);
     lym::k_batch_reduce<<<b->nconf, 256, 0, b->stream>>>(b->partials, b->d_desc, b->nconf, b->ntiles,
                                                         b->obs + size_t(4) * b->nconf * slot);

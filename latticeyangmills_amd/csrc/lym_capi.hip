@@ -143,7 +143,7 @@ template <typename S> int enqueue_observables(lym_ctx *c, int slot, hipEvent_t *
 {
     const lym::F2<S> *f = static_cast<const lym::F2<S> *>(c->W[c->cur]);
     if (ev) LYM_HIP(c, hipEventRecord(ev[0], c->stream));
-    lym::k_observables<S><<<c->nblk_obs, lym::kObsThreads, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
+    lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
     if (ev) LYM_HIP(c, hipEventRecord(ev[1], c->stream));
     lym::k_reduce_partials<<<1, lym::kReduceThreads, 0, c->stream>>>(c->partials, c->nblk_obs, c->obs + 4 * slot);
     LYM_LAUNCHED(c, "k_observables");

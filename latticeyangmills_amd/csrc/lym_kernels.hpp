@@ -355,6 +355,27 @@ template <typename R> LYM_DI R retr_ah(const ah8<R> &A, const ah8<R> &B)
     return -d - R(2) * o;
 }
 
+// Contributions of one plane pair at one site (src/Observables/superobs.cpp:40-124):
+// PAIR 0 = planes (01, 23), 1 = (02, 13), 2 = (03, 12).  out[0] = Re Tr P over
+// both planes, out[1] = Re Tr(C C) over both planes (the reference's raw
+// energy terms, superobs.cpp:109), out[2] = this pair's term of
+// q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124).
+template <typename R, typename S, int PAIR>
+LYM_DI void pair_observables(const F2<S> *__restrict__ f, const Geom g, long s, double out[3])
+{
+    constexpr int M1 = 0, N1 = PAIR + 1;
+    constexpr int M2 = (PAIR == 2) ? 1 : (PAIR == 0 ? 2 : 1), N2 = (PAIR == 0) ? 3 : (PAIR == 1 ? 3 : 2);
+    const long V = g.V;
+    Nbr n = neighbours(g, s);
+    R p1, p2;
+    ah8<R> A = clover<R, S, M1, N1>(f, V, s, n, &p1);
+    ah8<R> B = clover<R, S, M2, N2>(f, V, s, n, &p2);
+    out[0] = double(p1) + double(p2);
+    out[1] = double(retr_ah(A, A)) + double(retr_ah(B, B));
+    const double t = double(retr_ah(A, B));
+    out[2] = (PAIR == 1) ? t : -t;  // -Re Tr(C02 C13^+) = +Re Tr(C02 C13)
+}
+
 // Block reduction of 4 compensated sums; thread 0 returns the block total.
 template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
 {
@@ -376,70 +397,33 @@ template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
     }
 }
 
-// Observables of one 64-site run with 384 threads (src/Observables/superobs.cpp:31-128).
-// Wave w = 0..5 forms the clover of plane w in the reference's loop order,
-// (01, 02, 03, 12, 13, 23), with Re Tr P and Re Tr C^2 at each site, and
-// leaves C in LDS; after a barrier, wave p = 0..2 forms the term of
-// q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124)
-// of plane pair (p, 5 - p).  One clover per thread keeps the registers of
-// one clover live, not two.  v[] = {Re Tr P, Re Tr C^2, q term, |q term|}.
+// 192-thread workgroups: 64 consecutive sites x 3 plane pairs (wave = pair).
+// Per-block compensated partials of {sum Re Tr P, sum Re Tr C^2, sum q,
+// sum |q terms|}.
 constexpr int kObsSites = 64;
-constexpr int kObsThreads = 6 * kObsSites;
 
-template <typename R, typename S>
-LYM_DI void observables_block(const F2<S> *__restrict__ f, const Geom g, long s, bool active, ksum v[4])
-{
-    __shared__ R lds[6 * 9 * kObsSites];
-    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
-    R *mine = lds + w * 9 * kObsSites + (threadIdx.x & 63);
-    if (active) {
-        const Nbr n = neighbours(g, s);
-        R plaq;
-        ah8<R> C;
-        switch (w) {
-        case 0: C = clover<R, S, 0, 1>(f, g.V, s, n, &plaq); break;
-        case 1: C = clover<R, S, 0, 2>(f, g.V, s, n, &plaq); break;
-        case 2: C = clover<R, S, 0, 3>(f, g.V, s, n, &plaq); break;
-        case 3: C = clover<R, S, 1, 2>(f, g.V, s, n, &plaq); break;
-        case 4: C = clover<R, S, 1, 3>(f, g.V, s, n, &plaq); break;
-        default: C = clover<R, S, 2, 3>(f, g.V, s, n, &plaq); break;
-        }
-        v[0].s = double(plaq);
-        v[1].s = double(retr_ah(C, C));
-        const R c9[9] = {C.a0, C.a1, C.a2, C.o01.re, C.o01.im, C.o02.re, C.o02.im, C.o12.re, C.o12.im};
-#pragma unroll
-        for (int k = 0; k < 9; k++) mine[k * kObsSites] = c9[k];
-    }
-    __syncthreads();
-    if (active && w < 3) {
-        auto read = [&](int plane) {
-            const R *o = lds + plane * 9 * kObsSites + (threadIdx.x & 63);
-            ah8<R> C;
-            C.a0 = o[0];
-            C.a1 = o[kObsSites];
-            C.a2 = o[2 * kObsSites];
-            C.o01 = cplx<R>{o[3 * kObsSites], o[4 * kObsSites]};
-            C.o02 = cplx<R>{o[5 * kObsSites], o[6 * kObsSites]};
-            C.o12 = cplx<R>{o[7 * kObsSites], o[8 * kObsSites]};
-            return C;
-        };
-        const double t = double(retr_ah(read(w), read(5 - w)));
-        const double q = (w == 1) ? t : -t;  // -Re Tr(C02 C13^+) = +Re Tr(C02 C13)
-        v[2].s = q;
-        v[3].s = fabs(q);
-    }
-}
-
-// One partial {sum Re Tr P, sum Re Tr C^2, sum q, sum |q terms|} per 64-site run.
 template <typename S>
-__global__ __launch_bounds__(kObsThreads) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
-                                                             ksum *__restrict__ partials)
+__global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
+                                                     ksum *__restrict__ partials)
 {
-    const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
+    const long run = sched_run(sc, blockIdx.x);
     const long s = run * kObsSites + (threadIdx.x & 63);
+    const int pair = threadIdx.x >> 6;
     ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-    observables_block<S, S>(f, g, s, run >= 0 && s < g.V, v);
-    block_ksum4<kObsThreads>(v, partials + 4 * blockIdx.x);
+    if (run >= 0 && s < g.V) {
+        double o[3];
+        if (pair == 0)
+            pair_observables<S, S, 0>(f, g, s, o);
+        else if (pair == 1)
+            pair_observables<S, S, 1>(f, g, s, o);
+        else
+            pair_observables<S, S, 2>(f, g, s, o);
+        v[0].s = o[0];
+        v[1].s = o[1];
+        v[2].s = o[2];
+        v[3].s = fabs(o[2]);
+    }
+    block_ksum4<192>(v, partials + 4 * blockIdx.x);
 }
 
 // Sum nblk block partials into dst[0..3] (one workgroup of 1024 threads).

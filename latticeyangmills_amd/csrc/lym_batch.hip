@@ -119,13 +119,15 @@ static __global__ __launch_bounds__(256) void k_batch_reduce(const ksum *__restr
 }
 
 // checkerboard Metropolis over the batch: work item i of sub-sweep (mu, parity)
-// is half-site i - half0[c] of lattice c
+// is half-site i - half0[c] of lattice c.  Workgroup b takes block
+// (b % 8) * per_xcd + b / 8, so each XCD sweeps a contiguous range (as k_metropolis).
 template <typename S, int CAND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_batch_metropolis(F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
-                                                          int nconf, long nhalf, int mu, MetroParams p,
-                                                          unsigned long long *__restrict__ accepted)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_batch_metropolis(
+    F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc, int nconf, long nhalf, int mu, MetroParams p,
+    unsigned per_xcd, unsigned long long *__restrict__ accepted)
 {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const unsigned blk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const long i = (long)blk * 256 + threadIdx.x;
     int acc = 0, c = -1;
     if (i < nhalf) {
         int lo = 0, hi = nconf - 1;
@@ -144,7 +146,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ?
         F2<S> *fc = f + d.link_off;
         acc = metro_link_mu<S, CAND, false>(fc, d.g, mu, s, q);
     }
-    if (acc) atomicAdd(accepted + c, (unsigned long long)acc);
+    // Acceptance counts per lattice: one atomic per (wave, lattice) rather than
+    // per thread.  A wave spans consecutive work items, so it touches one
+    // lattice, or a few at a boundary: take the smallest pending lattice index,
+    // sum its lanes, repeat (the loop condition is uniform over the wave).
+    bool pending = c >= 0;
+    while (__any(pending)) {
+        int m = pending ? c : 0x7fffffff;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+        int v = (pending && c == m) ? acc : 0;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(accepted + m, (unsigned long long)v);
+        pending = pending && c != m;
+    }
 }
 
 }  // namespace lym
@@ -514,13 +530,14 @@ int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds)
 extern "C++" template <typename S>
 static void launch_batch_metropolis(lym_batch *b, unsigned nb, int mu, const lym::MetroParams &p)
 {
+    const unsigned per_xcd = (nb + 7) / 8;
     auto *f = static_cast<lym::F2<S> *>(b->W[b->cur]);
     if (p.candidate == LYM_CAND_SU2)
-        lym::k_batch_metropolis<S, 0><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+        lym::k_batch_metropolis<S, 0><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
     else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max)
-        lym::k_batch_metropolis<S, 1><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+        lym::k_batch_metropolis<S, 1><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
     else  // every candidate has c1 < 8 eps^2: series-only exponential
-        lym::k_batch_metropolis<S, 2><<<nb, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, b->accepted);
+        lym::k_batch_metropolis<S, 2><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
 }
 
 int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,

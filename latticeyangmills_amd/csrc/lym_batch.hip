@@ -255,16 +255,21 @@ void launch_stage(const lym_batch *b, const lym::F2<S> *win, lym::F2<S> *wout, S
 {
     constexpr bool re = sizeof(S) == 4;
     const unsigned g = stage_grid(b);
-    bool xrows = true;  // every lattice's runs hold whole x-rows
-    for (auto &d : b->dims) xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
-    const int lds = !(b->sched.kind & lym::SCHED_LDS) ? 0 : xrows ? 2 : 1;
+    bool xrows = true, slab = true;  // every lattice's runs hold whole x-rows / take the slab staging
+    for (auto &d : b->dims) {
+        xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
+        slab = slab && lym::slab_fits(d.data());
+    }
+    const int lds = !(b->sched.kind & lym::SCHED_LDS) ? 0 : slab ? 3 : xrows ? 2 : 1;
     auto go = [&](auto kern) { kern<<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf, b->ntiles, dev_sched(b), eps); };
     if (b->grid > 0) {
         if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, true>);
+        else if (lds == 3) go(lym::k_batch_flow_stage<S, STAGE, re, 3, true>);
         else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, true>);
         else go(lym::k_batch_flow_stage<S, STAGE, re, 1, true>);
     } else {
         if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, false>);
+        else if (lds == 3) go(lym::k_batch_flow_stage<S, STAGE, re, 3, false>);
         else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, false>);
         else go(lym::k_batch_flow_stage<S, STAGE, re, 1, false>);
     }

@@ -109,6 +109,8 @@ void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *
     const unsigned nb = stencil_blocks(c);
     if (!(c->sched.kind & lym::SCHED_LDS))
         lym::k_flow_stage<S, STAGE, reunit, 0><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+    else if (lym::slab_fits(c->dims))  // whole x-rows, never crossing a z-plane
+        lym::k_flow_stage<S, STAGE, reunit, 3><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
     else if (lym::kSitesPerBlock % c->dims[0] == 0)  // runs hold whole x-rows
         lym::k_flow_stage<S, STAGE, reunit, 2><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
     else

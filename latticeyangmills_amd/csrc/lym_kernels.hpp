@@ -29,46 +29,76 @@ template <typename S> using F2 = vec2_t<S>;
 
 // ---------------------------------------------------------- link sources ---
 
-// Where a link read of the staple lies relative to the workgroup's 64-site
-// run, known at compile time for each term:
-enum LinkWhere { ELSEWHERE = 0, XAXIS = 1, OWN = 2 };
-//   OWN       the site itself (always inside the run)
-//   XAXIS     a neighbour along x only (inside the run when runs hold whole
-//             x-rows, i.e. Lx divides 64)
-//   ELSEWHERE any other neighbour
+// A link read of the staple is tagged at compile time with its offset from
+// the site being updated: DX, DY in {-1, 0, 1} along x and y, and ZT = 1 when
+// the offset has a z or t part.  A link source decides from the tag alone
+// whether the read comes from LDS, so no lane ever branches on it.
 
 // Links of a field read straight from HBM (through L1/L2).
 template <typename S> struct GlobalLinks {
     const F2<S> *f;
     long V;
-    template <typename R, int WHERE> LYM_DI su3<R> get(int dir, long site) const
+    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
     {
         return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
     }
 };
 
+template <typename R, typename S> LYM_DI su3<R> lds_link(const F2<S> *p, int stride)
+{
+    su3<R> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const F2<S> v = p[k * stride];
+        m.e[k] = cplx<R>{R(v.x), R(v.y)};
+    }
+    return m;
+}
+
 // The four links of the run's 64 sites staged in LDS:
 // lds[(dir*9 + k)*64 + (site - site0)], consecutive lanes at consecutive
-// addresses (no bank conflicts).  OWN reads, and XAXIS reads when XROWS, come
-// from LDS; the choice is made at compile time, so no lane ever diverges.
-// Per link update 3 of the 19 staple reads are OWN and, with XROWS, another
-// 3 are XAXIS: a third of the vector-memory traffic moves to LDS.
+// addresses (no bank conflicts).  Reads at the site itself, and along x when
+// XROWS (runs hold whole x-rows, Lx | 64), come from LDS.  Per link update
+// 3 of the 19 staple reads are at the site itself and, with XROWS, another 3
+// are along x.
 template <typename S, bool XROWS> struct RunLinks {
     const F2<S> *f;
     long V;
     long site0;
     const F2<S> *lds;
-    template <typename R, int WHERE> LYM_DI su3<R> get(int dir, long site) const
+    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
     {
-        if constexpr (WHERE == OWN || (WHERE == XAXIS && XROWS)) {
-            const unsigned li = unsigned(site - site0);
-            su3<R> m;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const F2<S> v = lds[(dir * 9 + k) * kSitesPerBlock + li];
-                m.e[k] = cplx<R>{R(v.x), R(v.y)};
-            }
-            return m;
+        if constexpr (ZT == 0 && DY == 0 && (DX == 0 || XROWS))
+            return lds_link<R, S>(lds + dir * 9 * kSitesPerBlock + unsigned(site - site0), kSitesPerBlock);
+        else
+            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+    }
+};
+
+// Slab staging for runs of R = 64 / Lx whole x-rows (Lx | 64, Lx <= 32, and
+// Ly a multiple of R so that a run never crosses a z-plane): the run's rows
+// and one halo row on each side in y (periodic) are staged in LDS as
+// slab[(dir*9 + k)*kSlabSites + (row + 1)*Lx + x], row = -1 .. R.  Every read
+// whose offset has no z or t part comes from LDS: 10.5 of the 19 reads per
+// link update on average, against 7 with XROWS alone.
+constexpr int kSlabSites = 128;  // (R + 2) * Lx = 64 + 2 Lx <= 128
+
+// Whether a lattice takes the slab staging: Lx | 64, Lx <= 32, Ly % (64 / Lx) == 0.
+inline bool slab_fits(const int dims[4])
+{
+    const int lx = dims[0];
+    return lx >= 2 && lx <= 32 && kSitesPerBlock % lx == 0 && dims[1] % (kSitesPerBlock / lx) == 0;
+}
+template <typename S> struct SlabLinks {
+    const F2<S> *f;
+    long V;
+    const F2<S> *slab;
+    int lx_mask, row, x;  // Lx - 1; this lane's row in the run and x coordinate
+    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
+    {
+        if constexpr (ZT == 0) {
+            const int li = (row + 1 + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
+            return lds_link<R, S>(slab + dir * 9 * kSlabSites + li, kSlabSites);
         } else {
             return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
         }
@@ -87,43 +117,42 @@ template <typename S, bool XROWS> struct RunLinks {
 template <typename R, int MU, int NU, typename Src>
 LYM_DI void staple_plane(const Src &src, long s, const Nbr &n, su3<R> &om)
 {
-    constexpr int AMU = (MU == 0) ? XAXIS : ELSEWHERE;  // x + MU
-    constexpr int ANU = (NU == 0) ? XAXIS : ELSEWHERE;  // x + NU, x - NU
+    // offset tags (unit vectors of MU and NU split into x, y and z/t parts)
+    constexpr int MX = MU == 0, MY = MU == 1, MZ = MU >= 2, NX = NU == 0, NY = NU == 1, NZ = NU >= 2;
+    const long y = s + n.bwd[NU];  // x - NU
     if constexpr (sizeof(R) == 4) {
         {
-            su3<R> a = src.template get<R, AMU>(NU, s + n.fwd[MU]);
-            su3<R> b = src.template get<R, ANU>(MU, s + n.fwd[NU]);
+            su3<R> a = src.template get<R, MX, MY, MZ>(NU, s + n.fwd[MU]);
+            su3<R> b = src.template get<R, NX, NY, NZ>(MU, s + n.fwd[NU]);
             su3<R> t, u;
             mul_na_rows01(t, a, b);
-            su3<R> c = src.template get<R, OWN>(NU, s);
+            su3<R> c = src.template get<R, 0, 0, 0>(NU, s);
             mul_na_rows01(u, t, c);
             acc_su3_rows01(om, u);
         }
         {
-            const long y = s + n.bwd[NU];
-            su3<R> a = src.template get<R, ANU>(MU, y);
-            su3<R> b = src.template get<R, ELSEWHERE>(NU, y + n.fwd[MU]);
+            su3<R> a = src.template get<R, -NX, -NY, NZ>(MU, y);
+            su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ>(NU, y + n.fwd[MU]);
             su3<R> t, u;
             mul_nn_cols01(t, a, b);  // (a b)^+ rows 0, 1 need columns 0, 1 of a b
-            su3<R> c = src.template get<R, ANU>(NU, y);
+            su3<R> c = src.template get<R, -NX, -NY, NZ>(NU, y);
             mul_an_rows01(u, t, c);
             acc_su3_rows01(om, u);
         }
         return;
     }
     {
-        su3<R> a = src.template get<R, AMU>(NU, s + n.fwd[MU]);
-        su3<R> b = src.template get<R, ANU>(MU, s + n.fwd[NU]);
+        su3<R> a = src.template get<R, MX, MY, MZ>(NU, s + n.fwd[MU]);
+        su3<R> b = src.template get<R, NX, NY, NZ>(MU, s + n.fwd[NU]);
         su3<R> t = mul_na(a, b);
-        su3<R> c = src.template get<R, OWN>(NU, s);
+        su3<R> c = src.template get<R, 0, 0, 0>(NU, s);
         acc_na(om, t, c);
     }
     {
-        const long y = s + n.bwd[NU];  // same MU-coordinate as s
-        su3<R> a = src.template get<R, ANU>(MU, y);
-        su3<R> b = src.template get<R, ELSEWHERE>(NU, y + n.fwd[MU]);
+        su3<R> a = src.template get<R, -NX, -NY, NZ>(MU, y);
+        su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ>(NU, y + n.fwd[MU]);
         su3<R> t = mul_nn(a, b);
-        su3<R> c = src.template get<R, ANU>(NU, y);
+        su3<R> c = src.template get<R, -NX, -NY, NZ>(NU, y);
         acc_an(om, t, c);
     }
 }
@@ -167,7 +196,7 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     const long V = g.V;
     Nbr n = neighbours(g, s);
     su3<S> om = staple_from<S, MU>(src, s, n);
-    su3<S> U = src.template get<S, OWN>(MU, s);
+    su3<S> U = src.template get<S, 0, 0, 0>(MU, s);
     su3<S> X = mul_nn(U, om);
     herm8<double> z = force_from_x(X);
     S *am = acc + (long)MU * 8 * V;
@@ -190,12 +219,16 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     store_link<S, double>(wout + (long)MU * 9 * V, V, unsigned(s), W);
 }
 
-// One 64-site run, all four directions (wave = direction).  LDS != 0 stages
-// the run's own links in LDS first (LDS == 2: runs hold whole x-rows, so
-// x-axis neighbours are read from LDS too) (every wave loads its direction's 64 links
-// and shares them); all 256 threads reach the barrier, then lanes past the end
-// of the lattice drop out.  The staging buffer is free again when the
-// function returns (trailing barrier), so a persistent caller may loop.
+// One 64-site run, all four directions (wave = direction).  LDS selects what
+// is staged in LDS before the staples (each wave stages its own direction):
+//   0  nothing
+//   1  the run's own links (RunLinks, reads at the site itself)
+//   2  the same, runs holding whole x-rows (reads along x too)
+//   3  the run's rows plus one halo row on each side in y (SlabLinks, every
+//      read without a z or t part)
+// All 256 threads reach the barrier, then lanes past the end of the lattice
+// drop out.  The staging buffer is free again when the function returns
+// (trailing barrier), so a persistent caller may loop.
 template <typename S, int STAGE, bool REUNIT, int LDS>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
                        const Geom g, long site0, double eps)
@@ -204,7 +237,47 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
     const int lane = threadIdx.x & 63;
     const long s = site0 + lane;
     const bool active = s < g.V;
-    if constexpr (LDS != 0) {
+    if constexpr (LDS == 3) {
+        // Slab: runs are whole x-rows and never cross a z-plane, so every lane
+        // is a lattice site (checked on the host).  Wave mu stages direction mu:
+        // the run's rows at slab rows 1..R, and the rows y0 - 1 and y0 + R
+        // (periodic in y) at slab rows 0 and R + 1.
+        __shared__ F2<S> slab[4 * 9 * kSlabSites];
+        const int Lx = g.L[0], Ly = g.L[1], R = kSitesPerBlock / Lx;
+        const int y0 = int(uint32_t(site0) / uint32_t(Lx) % uint32_t(Ly));
+        const long lo = (y0 > 0) ? site0 - Lx : site0 - Lx + long(Ly) * Lx;
+        const long hi = (y0 + R < Ly) ? site0 + kSitesPerBlock : site0 + kSitesPerBlock - long(Ly) * Lx;
+        const F2<S> *plane = win + (long)mu * 9 * g.V;
+        F2<S> *dst = slab + mu * 9 * kSlabSites;
+        {
+            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(s));
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                dst[k * kSlabSites + Lx + lane].x = u.e[k].re;
+                dst[k * kSlabSites + Lx + lane].y = u.e[k].im;
+            }
+        }
+        if (lane < 2 * Lx) {
+            const bool below = lane < Lx;
+            const long hs = below ? lo + lane : hi + (lane - Lx);
+            const int li = below ? lane : (R + 1) * Lx + (lane - Lx);
+            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(hs));
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                dst[k * kSlabSites + li].x = u.e[k].re;
+                dst[k * kSlabSites + li].y = u.e[k].im;
+            }
+        }
+        __syncthreads();
+        const SlabLinks<S> src{win, g.V, slab, Lx - 1, lane >> __builtin_ctz(unsigned(Lx)), lane & (Lx - 1)};
+        switch (mu) {
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        }
+        __syncthreads();
+    } else if constexpr (LDS != 0) {
         __shared__ F2<S> lds[4 * 9 * kSitesPerBlock];
         if (active) {
             const su3<S> u = load_link<S, S>(win + (long)mu * 9 * g.V, g.V, unsigned(s));

@@ -127,3 +127,27 @@ def test_autotune_keeps_results(inputs, tmp_path):
         assert np.allclose(got[..., [0, 1, 3]], ref[..., [0, 1, 3]], rtol=1e-7, atol=0)
         assert np.allclose(got[..., 2], ref[..., 2], rtol=0, atol=1e-5)
         assert "cached" in b.autotune(cache)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("grid", [0, 8])
+def test_slab_staging_batch(precision, grid):
+    """Every lattice fits the slab staging (whole x-rows, y halo in LDS)."""
+    dims = [(8, 8, 2, 2), (16, 4, 2, 2), (32, 2, 2, 4)]
+    inputs = [orc.hot_start(d, 60 + i) for i, d in enumerate(dims)]
+    singles = []
+    for d, L in zip(dims, inputs):
+        with LatticeContext(d, precision) as ctx:
+            ctx.set_schedule("linear", 0)
+            ctx.upload(L)
+            singles.append((ctx.flow(0.02, 0.1, 1), ctx.download()))
+    with BatchContext(dims, precision) as b:
+        b.set_schedule(grid, "contig", 0, True)
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        series = b.flow(0.02, 0.1, 1, use_graph=True)
+        fp64 = precision == "fp64"
+        for c in range(len(dims)):
+            s, links = singles[c]
+            assert np.abs(b.download(c) - links).max() < (1e-13 if fp64 else 2e-6)
+            assert np.allclose(series[c][:, [0, 1, 3]], s[:, [0, 1, 3]], rtol=1e-13 if fp64 else 1e-7, atol=0)

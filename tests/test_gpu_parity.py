@@ -250,3 +250,25 @@ def test_check_links_reports_health(precision, tol):
         assert abs(dev - 0.331) < 1e-5
         ctx.set_cold()
         assert ctx.check_links() == (0.0, 0)
+
+
+@pytest.mark.parametrize("dims", [(32, 4, 2, 2), (16, 8, 2, 4), (8, 8, 2, 2), (4, 16, 2, 2), (2, 32, 2, 2),
+                                  (16, 4, 4, 2)])
+@pytest.mark.parametrize("kind", ["linear", "contig"])
+def test_slab_staging(dims, kind):
+    """Runs of whole x-rows with a y halo staged in LDS (Lx | 64, Lx <= 32,
+    Ly a multiple of 64 / Lx), including Ly = 64 / Lx where both halo rows are
+    the run's own rows: one FP64 step equals the oracle, and the mixed path
+    stays within FP32 rounding of it."""
+    links = orc.hot_start(dims, 41)
+    want = orc.flow_step(links, dims, 0.02)
+    with LatticeContext(dims) as ctx:
+        ctx.set_schedule(kind, 0, True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        assert np.max(np.abs(ctx.download() - want)) < 1e-12
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.set_schedule(kind, 0, True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        assert np.max(np.abs(ctx.download() - want)) < 2e-6

@@ -39,7 +39,7 @@ FLOP_PER_LINK_UPDATE = 3 * 3300  # per stage ~16.5 matmuls of 198 flop + exp (se
 # scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
 # back); the run, its commands and the per-stage figures are in PROFILE_FILE.
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
-PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.312e9, ("mixed-32x64", "fp64"): 5.514e9}
+PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.364e9, ("mixed-32x64", "fp64"): 6.896e9}
 
 WORKLOADS = {
     # 1000 sweeps from a Haar-random start put t0 (t^2 <E> = 0.3) near t = 5.1,

@@ -58,25 +58,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def e_mean(energy_raw, V):
-    """<E> from the reference's raw clover sum: C = 8 G, E = -sum Re Tr C^2 / 64 per site."""
-    return -energy_raw / (64.0 * V)
-
-
-def t0_from_series(series, V, target=0.3):
-    """t0 where t^2 <E> crosses `target` (linear interpolation), or None."""
-    prev = None
-    for t, plaq, topc, en in series:
-        if not math.isfinite(plaq):
-            continue
-        y = t * t * e_mean(en, V)
-        if prev is not None and prev[1] < target <= y:
-            t1, y1 = prev
-            return t1 + (target - y1) * (t - t1) / (y - y1)
-        prev = (t, y)
-    return None
-
-
 def timed_flow(ctx, eps, tau, every):
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -258,7 +239,10 @@ def main():
         rows = [i for i in range(len(s64)) if math.isfinite(s64[i, 1])]
         e_err = max(abs(sm[i, 3] / s64[i, 3] - 1.0) for i in rows)
         q_err = max(abs(sm[i, 2] - s64[i, 2]) for i in rows)
-        t0m, t064 = t0_from_series(sm, V), t0_from_series(s64, V)
+        from latticeyangmills_amd import scale
+        t0m, t064 = scale.t0(sm, V), scale.t0(s64, V)
+        w0m, w064 = scale.w0(sm, V), scale.w0(s64, V)
+        a_fm = scale.lattice_spacing_fm(w["beta"])
         last = rows[-1]
         out["fp64"] = {"value": lups64, "ms_per_step": f["ms_per_step"], "speedup_mixed": lups / lups64,
                        "stage_ms": [float(x) for x in f["stage_ms"][:3]], "schedule": f["tuner"]}
@@ -266,7 +250,10 @@ def main():
             "max_rel_err_E": e_err, "max_abs_err_Q": q_err,
             "t0_a2_mixed": t0m, "t0_a2_fp64": t064,
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),
-            "t2E_at_tmax": s64[last, 0] ** 2 * e_mean(s64[last, 3], V), "Q_at_tmax": s64[last, 2],
+            "w0_a_mixed": w0m, "w0_a_fp64": w064,
+            "w0_rel_err": (abs(w0m / w064 - 1) if (w0m and w064) else None),
+            "a_fm": a_fm, "t0_fm2": (t064 * a_fm * a_fm if t064 else None),
+            "t2E_at_tmax": s64[last, 0] ** 2 * scale.e_mean(s64[last, 3], V), "Q_at_tmax": s64[last, 2],
         }
     if not args.no_batch:
         out["batched"] = batched_ensemble(args.tune_cache)

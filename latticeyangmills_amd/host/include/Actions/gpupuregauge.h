@@ -1,12 +1,17 @@
 // Wilson plaquette action on the GPU: the drop-in for PureGauge
 // (include/Actions/puregauge.h:50-69, src/Actions/puregauge.cpp).
 #pragma once
+#include "Utils/latticeunits.h"
 #include "Actions/action.h"
 #include "Math/devicefield.h"
 
 class GpuPureGauge : public Action {
 public:
-    explicit GpuPureGauge(double beta, int precision = LYM_PREC_FP64) : m_beta(beta), m_precision(precision) {}
+    /// Sets LatticeUnits for beta, as the reference's constructor does (src/Actions/puregauge.cpp:24)
+    explicit GpuPureGauge(double beta, int precision = LYM_PREC_FP64) : m_beta(beta), m_precision(precision)
+    {
+        LatticeUnits::initialize(beta);
+    }
 
     /// -(beta/3) Re Tr[(newLink - U) Omega] with Omega the staple of the link
     /// from the last computeStaples(mu) (src/Actions/puregauge.cpp:28)

@@ -1,5 +1,6 @@
 // App interface of the reference (include/Apps/app.h:59-77).
 #pragma once
+#include "Utils/latticeunits.h"
 #include <array>
 #include <vector>
 
@@ -15,6 +16,7 @@ public:
     {
         m_size = latticeSize;
         m_lat = new GluonField(m_size);
+        LatticeUnits::initialize(LatticeUnits::beta, m_size);  // the volume is known from here on
     }
     void addObservable(Observable *observable) { m_obs.push_back(observable); }
     virtual void execute() = 0;

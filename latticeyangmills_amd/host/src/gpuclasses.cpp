@@ -8,7 +8,9 @@
 #include "Apps/gpugaugefieldfactory.h"
 #include "Apps/gpuwilsonflow.h"
 #include "InputOutput/confio.h"
+#include "Observables/flowscale.h"
 #include "Observables/gpusuperobs.h"
+#include "Utils/latticeunits.h"
 
 static_assert(sizeof(SU3) == 18 * sizeof(double), "SU3 must be 18 packed doubles for the C ABI");
 
@@ -115,6 +117,14 @@ std::vector<std::array<double, 4>> GpuWilsonFlow::applyWilsonFlow(int confNum)
     if (h.nonFinite > 0 || h.maxDeviation > allowed)
         fprintf(stderr, "warning: conf %d after the flow: %lld links non-finite, max deviation from SU(3) %.3g\n",
                 confNum, (long long)h.nonFinite, h.maxDeviation);
+    // gradient-flow scales of this configuration, in lattice and physical units
+    const FlowScales sc = flowScales(series, long(LatticeUnits::latticeSites));
+    if (LatticeUnits::beta > 0)
+        printf("conf %d: t0/a^2 = %.8g, w0/a = %.8g (a = %.5g fm at beta %g: t0 = %.5g fm^2, w0 = %.5g fm)\n", confNum,
+               sc.t0, sc.w0, LatticeUnits::latticeSpacing, LatticeUnits::beta,
+               sc.t0 * LatticeUnits::latticeSpacing * LatticeUnits::latticeSpacing, sc.w0 * LatticeUnits::latticeSpacing);
+    else
+        printf("conf %d: t0/a^2 = %.8g, w0/a = %.8g\n", confNum, sc.t0, sc.w0);
     if (!series.empty()) {
         // the observables hold the values at the last measured row
         for (auto *o : m_obs) {

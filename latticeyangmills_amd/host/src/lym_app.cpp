@@ -10,6 +10,9 @@
 //                              InputDir, OutputDir; optional build keys
 //                              "precision", "measureEvery", "seed", "nHit")
 //   lym_app io-selftest DIR   (host only: writes and reads back a .bin file)
+//   lym_app scales FILE VOLUME [BETA]
+//                             (host only: t0/a^2 and w0/a of a flow series file
+//                              flow/confNNNN.dat, and in fm when BETA is given)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -21,7 +24,9 @@
 #include "Apps/gpuwilsonflow.h"
 #include "InputOutput/confio.h"
 #include "InputOutput/jsonlite.h"
+#include "Observables/flowscale.h"
 #include "Observables/gpusuperobs.h"
+#include "Utils/latticeunits.h"
 
 namespace {
 struct Args {
@@ -34,7 +39,7 @@ struct Args {
 
 int usage()
 {
-    fprintf(stderr, "usage: lym_app run INPUT.json | flow ... | generate ... | io-selftest DIR\n"
+    fprintf(stderr, "usage: lym_app run INPUT.json | flow ... | generate ... | scales FILE VOLUME [BETA] | io-selftest DIR\n"
                     "       (options are listed at the top of lym_app.cpp)\n");
     return 2;
 }
@@ -134,6 +139,22 @@ int main(int argc, char **argv)
         if (cmd == "run") {
             if (argc < 3) return usage();
             return run_json(argv[2]);
+        }
+        if (cmd == "scales") {
+            if (argc < 4) return usage();
+            std::vector<std::array<double, 4>> rows;
+            FILE *f = fopen(argv[2], "r");
+            if (!f) throw std::runtime_error(std::string("cannot read ") + argv[2]);
+            std::array<double, 4> r;
+            while (fscanf(f, "%lf %lf %lf %lf", &r[0], &r[1], &r[2], &r[3]) == 4) rows.push_back(r);
+            fclose(f);
+            const FlowScales sc = flowScales(rows, atol(argv[3]));
+            printf("t0/a^2 = %.12g\nw0/a = %.12g\n", sc.t0, sc.w0);
+            if (argc > 4) {
+                const double a = LatticeUnits::calculateLatticeSpacing(atof(argv[4]));
+                printf("a = %.12g fm\nt0 = %.12g fm^2\nw0 = %.12g fm\n", a, sc.t0 * a * a, sc.w0 * a);
+            }
+            return 0;
         }
         if (cmd == "io-selftest") {
             if (argc < 3) return usage();

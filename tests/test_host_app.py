@@ -97,3 +97,19 @@ def test_json_front_end_pgfg_then_flow(tmp_path):
     want, _ = orc.flow(read_bin(confs / "conf0001.bin", dims), dims, 0.02, 0.2, 1, 0)
     assert np.all(np.abs(got[:, 1] / want[:, 1] - 1) < 1e-12)
     assert np.all(np.abs(got[:, 3] / want[:, 3] - 1) < 1e-12)
+
+
+def test_scales_of_a_flow_series(tmp_path):
+    """lym_app scales (host C++ Observables/flowscale.h, Utils/latticeunits.h)
+    agrees with the closed forms and with latticeyangmills_amd.scale."""
+    import math
+    from latticeyangmills_amd import scale
+    V, c = 4096, 0.06
+    rows = [[t, 0.9, 0.0, -64.0 * V * (c / t if t > 0 else 0.0)] for t in np.arange(0, 8.0, 0.02)]
+    f = tmp_path / "conf0000.dat"
+    f.write_text("".join("%.17g %.17g %.17g %.17g\n" % tuple(r) for r in rows))
+    out = dict(line.split(" = ") for line in run("scales", f, V, 6.2).splitlines())
+    assert abs(float(out["t0/a^2"]) - 0.3 / c) < 1e-9
+    assert abs(float(out["w0/a"]) - math.sqrt(0.3 / c)) < 1e-9
+    assert abs(float(out["a"].split()[0]) - scale.lattice_spacing_fm(6.2)) < 1e-12
+    assert abs(float(out["t0"].split()[0]) - 0.3 / c * scale.lattice_spacing_fm(6.2) ** 2) < 1e-12

@@ -2,6 +2,7 @@
 #include <cstdio>
 #include <map>
 #include <memory>
+#include <mutex>
 
 #include "Math/devicefield.h"
 #include "Math/lattice.h"
@@ -80,18 +81,30 @@ std::map<GluonField *, std::unique_ptr<DeviceField>> &registry()
     static std::map<GluonField *, std::unique_ptr<DeviceField>> r;
     return r;
 }
+// Guards the registry only: a DeviceField itself, like its lym_ctx, is used
+// from one thread at a time.
+std::mutex &registry_lock()
+{
+    static std::mutex m;
+    return m;
+}
 }  // namespace
 
 DeviceField &DeviceField::of(GluonField *field, int precision, int device)
 {
     if (!field) throw std::runtime_error("DeviceField: null GluonField");
+    std::lock_guard<std::mutex> g(registry_lock());
     auto &r = registry();
     auto it = r.find(field);
     if (it == r.end()) it = r.emplace(field, std::unique_ptr<DeviceField>(new DeviceField(field, precision, device))).first;
     return *it->second;
 }
 
-void DeviceField::release(GluonField *field) { registry().erase(field); }
+void DeviceField::release(GluonField *field)
+{
+    std::lock_guard<std::mutex> g(registry_lock());
+    registry().erase(field);
+}
 
 DeviceField::DeviceField(GluonField *field, int precision, int device) : m_host(field), m_precision(precision)
 {

@@ -68,7 +68,8 @@ class BatchContext:
 
     def upload(self, c, links):
         links = np.ascontiguousarray(links, dtype=np.float64)
-        assert links.shape == (4, self.volumes[c], 18)
+        if links.shape != (4, self.volumes[c], 18):
+            raise ValueError(f"links of lattice {c} must have shape (4, {self.volumes[c]}, 18), not {links.shape}")
         self._check(self._lib.lym_batch_upload_links(self._h, c, _links_ptrs(links)), "lym_batch_upload_links")
 
     def download(self, c):

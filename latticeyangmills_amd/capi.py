@@ -125,7 +125,8 @@ class LatticeContext:
     # --- field transfer ---------------------------------------------------
     def upload(self, links):
         links = np.ascontiguousarray(links, dtype=np.float64)
-        assert links.shape == (4, self.volume, 18), links.shape
+        if links.shape != (4, self.volume, 18):
+            raise ValueError(f"links must have shape (4, {self.volume}, 18), not {links.shape}")
         self._check(self._lib.lym_upload_links(self._h, _links_ptrs(links)), "lym_upload_links")
 
     def download(self):
@@ -202,7 +203,9 @@ class LatticeContext:
         if predrawn is not None:
             pd = np.ascontiguousarray(predrawn, dtype=np.float64)
             need = nsweeps * 4 * self.volume * nhit * UNIFORMS_PER_HIT[candidate]
-            assert pd.size == need, (pd.size, need)
+            if pd.size != need:
+                raise ValueError(f"predrawn must hold nsweeps * 4V * nhit * {UNIFORMS_PER_HIT[candidate]} = {need} "
+                                 f"uniforms, not {pd.size}")
         self._check(self._lib.lym_metropolis(self._h, beta, eps, nhit, nsweeps, first_sweep, CANDIDATES[candidate],
                                              seed, _dptr(pd) if pd is not None else None, ctypes.byref(acc)),
                     "lym_metropolis")

@@ -28,3 +28,15 @@ def test_exports_every_declared_symbol(header):
 def test_package_binding_loads():
     from latticeyangmills_amd import lib
     assert lib() is not None
+
+
+def test_bindings_reject_wrong_shapes_without_a_gpu():
+    """Shape checks raise before any call reaches the library (they are not asserts)."""
+    import numpy as np
+    from latticeyangmills_amd.capi import LatticeContext
+    ctx = LatticeContext.__new__(LatticeContext)  # no device: only the Python-side checks run
+    ctx.volume = 16
+    with pytest.raises(ValueError, match="shape"):
+        LatticeContext.upload(ctx, np.zeros((4, 15, 18)))
+    with pytest.raises(ValueError, match="uniforms"):
+        LatticeContext.metropolis(ctx, 6.0, 0.24, 2, 1, 0, "exph", 0, np.zeros(10))

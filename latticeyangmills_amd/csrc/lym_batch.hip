@@ -35,6 +35,7 @@ struct ConfDesc {
     long tile0;     // first tile of the lattice
     long half0;     // first Metropolis work item of the lattice (V/2 sites per sub-sweep)
     uint64_t seed;
+    int slab;       // the lattice fits the slab staging (slab_fits)
 };
 
 constexpr int kMaxConf = 1024;
@@ -54,6 +55,9 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
 
 // PERSIST = false: one workgroup per schedule slot (no loop, so nothing is
 // kept live across tiles); true: a persistent grid strides over the slots.
+// LDS as flow_block, and 4: per lattice, slab staging where the lattice fits
+// it and the run's own links elsewhere (the lattice of a tile, and so the
+// choice, is uniform over the workgroup).
 template <typename S, int STAGE, bool REUNIT, int LDS, bool PERSIST>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
@@ -65,8 +69,15 @@ __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restric
         if (tile >= 0) {
             const int c = find_conf(desc, nconf, tile);
             const ConfDesc d = desc[c];
-            flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g,
-                                              (tile - d.tile0) * kSitesPerBlock, eps);
+            const long site0 = (tile - d.tile0) * kSitesPerBlock;
+            if constexpr (LDS == 4) {
+                if (d.slab)
+                    flow_block<S, STAGE, REUNIT, 3>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+                else
+                    flow_block<S, STAGE, REUNIT, 1>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+            } else {
+                flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+            }
         }
         if constexpr (!PERSIST) break;
     }
@@ -260,15 +271,19 @@ void launch_stage(const lym_batch *b, const lym::F2<S> *win, lym::F2<S> *wout, S
         xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
         slab = slab && lym::slab_fits(d.data());
     }
-    const int lds = !(b->sched.kind & lym::SCHED_LDS) ? 0 : slab ? 3 : xrows ? 2 : 1;
+    bool any_slab = false;
+    for (auto &d : b->dims) any_slab = any_slab || lym::slab_fits(d.data());
+    const int lds = !(b->sched.kind & lym::SCHED_LDS) ? 0 : slab ? 3 : any_slab ? 4 : xrows ? 2 : 1;
     auto go = [&](auto kern) { kern<<<g, 256, 0, b->stream>>>(win, wout, acc, b->d_desc, b->nconf, b->ntiles, dev_sched(b), eps); };
     if (b->grid > 0) {
         if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, true>);
+        else if (lds == 4) go(lym::k_batch_flow_stage<S, STAGE, re, 4, true>);
         else if (lds == 3) go(lym::k_batch_flow_stage<S, STAGE, re, 3, true>);
         else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, true>);
         else go(lym::k_batch_flow_stage<S, STAGE, re, 1, true>);
     } else {
         if (lds == 0) go(lym::k_batch_flow_stage<S, STAGE, re, 0, false>);
+        else if (lds == 4) go(lym::k_batch_flow_stage<S, STAGE, re, 4, false>);
         else if (lds == 3) go(lym::k_batch_flow_stage<S, STAGE, re, 3, false>);
         else if (lds == 2) go(lym::k_batch_flow_stage<S, STAGE, re, 2, false>);
         else go(lym::k_batch_flow_stage<S, STAGE, re, 1, false>);
@@ -387,6 +402,7 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
             return LYM_ERR_ARG;
         }
         d.g.V = V;
+        d.slab = lym::slab_fits(d.g.L) ? 1 : 0;
         d.link_off = links * 9;
         d.acc_off = links * 8;
         d.tile0 = tiles;

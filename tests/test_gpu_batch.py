@@ -131,9 +131,11 @@ def test_autotune_keeps_results(inputs, tmp_path):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("grid", [0, 8])
-def test_slab_staging_batch(precision, grid):
-    """Every lattice fits the slab staging (whole x-rows, y halo in LDS)."""
-    dims = [(8, 8, 2, 2), (16, 4, 2, 2), (32, 2, 2, 4)]
+@pytest.mark.parametrize("dims", [[(8, 8, 2, 2), (16, 4, 2, 2), (32, 2, 2, 4)],
+                                  [(8, 8, 2, 2), (6, 4, 4, 2), (16, 4, 2, 2), (12, 2, 2, 2)]])
+def test_slab_staging_batch(precision, grid, dims):
+    """All lattices fit the slab staging (whole x-rows, y halo in LDS), or only
+    some do and each workgroup picks the staging of its lattice."""
     inputs = [orc.hot_start(d, 60 + i) for i, d in enumerate(dims)]
     singles = []
     for d, L in zip(dims, inputs):

@@ -82,10 +82,16 @@ def test_cfg2_whole_flow_properties():
 
 
 def test_cfg3_mixed_within_stated_error():
-    """32^3x64, beta 6.2: mixed flow vs FP64 flow of the same config to t = 5.98."""
+    """32^3x64, beta 6.2 (Haar start, 1000 sweeps, seed 62, as the bench): mixed
+    flow vs FP64 flow of the same config to t = 5.98.  Bounds: <E> and the
+    plaquette 1e-6 relative at every measured row, t0/a^2 1e-6 relative,
+    Q 1e-4 absolute."""
+    from latticeyangmills_amd import scale
     dims = (32, 32, 32, 64)
+    V = orc.volume(dims)
     with LatticeContext(dims) as ctx:
-        ctx.metropolis(6.2, 0.24, 10, 50, 0, "exph", 62)
+        ctx.set_hot(62)
+        ctx.metropolis(6.2, 0.24, 10, 1000, 0, "exph", 62)
         links = ctx.download()
         s64 = ctx.flow(0.02, 6.0, 10)
     with LatticeContext(dims, "mixed") as ctx:
@@ -96,6 +102,11 @@ def test_cfg3_mixed_within_stated_error():
     assert max(rel(smx[r, 3], s64[r, 3]) for r in rows) < 1e-6
     assert max(abs(smx[r, 2] - s64[r, 2]) for r in rows) < 1e-4
     assert max(rel(smx[r, 1], s64[r, 1]) for r in rows) < 1e-6
+    t0m, t064 = scale.t0(smx, V), scale.t0(s64, V)
+    assert t064 is not None and 4.0 < t064 < 6.0  # reached inside the flow (5.09 on this config)
+    assert rel(t0m, t064) < 1e-6
+    w0m, w064 = scale.w0(smx, V), scale.w0(s64, V)
+    assert (w0m is None) == (w064 is None) and (w064 is None or rel(w0m, w064) < 1e-6)
     assert u < 1e-6  # FP32 storage of reunitarised links
 
 

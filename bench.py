@@ -33,7 +33,12 @@ VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 78.6}  # non-packed FP32 issues at the
 # read + written (stage 2), read (stage 3): 140 reals.
 REALS_PER_LINK_UPDATE = 140
 STAGE_REALS = (44, 52, 44)
-FLOP_PER_LINK_UPDATE = 3 * 3300  # per stage ~16.5 matmuls of 198 flop + exp
+# VALU arithmetic per link-update (three RK stages), from the rocprofv3 VALU
+# instruction counters of the stage kernels at 32^3x64 (64 lanes x (ADD + MUL
+# + 2 FMA), FP32 and FP64 summed; per-stage figures in PROFILE_FILE): the
+# mixed path forms its staples from two rows (2.82 kflop per link per stage),
+# FP64 multiplies out all three (3.21 kflop).
+FLOP_PER_LINK_UPDATE = {"mixed": 8460, "fp64": 9620}
 # HBM-side bytes per k_flow_stage launch, averaged over the three RK stages,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this bench (fetch
 # scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
@@ -229,7 +234,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_step / 3,
                      "kernel": "k_flow_stage (RK stages 1+2+3 per step)",
                      "stage_ms": [float(x) for x in st[:3]], "observables_ms": float(st[3]),
-                     "valu_tflops": 4 * V * FLOP_PER_LINK_UPDATE / (step_kernel_ms * 1e-3) / 1e12,
+                     "valu_tflops": 4 * V * FLOP_PER_LINK_UPDATE[main_prec] / (step_kernel_ms * 1e-3) / 1e12,
+                     "valu_peak_tflops": VALU_PEAK_TFLOPS["fp32" if main_prec == "mixed" else "fp64"],
+                     "valu_frac": 4 * V * FLOP_PER_LINK_UPDATE[main_prec] / (step_kernel_ms * 1e-3) / 1e12
+                     / VALU_PEAK_TFLOPS["fp32" if main_prec == "mixed" else "fp64"],
                      "schedule": r["tuner"]},
     }
     if "fp64" in results and main_prec != "fp64":

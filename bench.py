@@ -198,7 +198,16 @@ def main():
     results = {}
     for prec in ([w["precision"], "fp64"] if w["precision"] != "fp64" else ["fp64"]):
         with LatticeContext(dims, prec) as ctx:
+            # host <-> device transfer of the configuration at the boundary
+            # (GluonField layout in host memory), for the transfer-inclusive rate
+            ctx.synchronize()
+            t = time.perf_counter()
             ctx.upload(links)
+            ctx.synchronize()
+            up_s = time.perf_counter() - t
+            t = time.perf_counter()
+            ctx.download()
+            down_s = time.perf_counter() - t
             tuner = ctx.autotune(args.tune_cache)
             if args.warmup:
                 ctx.flow_steps(eps, args.warmup)
@@ -207,7 +216,8 @@ def main():
             series, el = timed_flow(ctx, eps, tau, every)
             ctx.upload(links)
             total_ms, st = ctx.profile_flow(eps, args.profile_steps, every)
-        results[prec] = dict(series=series, seconds=el, stage_ms=st, ms_per_step=1e3 * el / steps, tuner=tuner)
+        results[prec] = dict(series=series, seconds=el, stage_ms=st, ms_per_step=1e3 * el / steps, tuner=tuner,
+                             upload_s=up_s, download_s=down_s)
         log(f"[bench] {prec}: {steps} steps in {el:.3f} s ({1e3 * el / steps:.3f} ms/step), stage ms {st}, {tuner}")
 
     main_prec = w["precision"]
@@ -225,6 +235,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if main_prec == "mixed" else "f64",
         "data": "synthetic: Philox-seeded checkerboard Metropolis thermalised on the GPU",
+        "transfers": {"upload_ms": 1e3 * r["upload_s"], "download_ms": 1e3 * r["download_s"],
+                      "link_updates_per_s_with_transfers": 4 * V * steps / (r["seconds"] + r["upload_s"] + r["download_s"]),
+                      "note": "one upload and one download of the configuration (GluonField layout in host memory) "
+                              "around the timed flow; value excludes them"},
         "config": {"workload": args.workload, "lattice": "x".join(map(str, dims)), "beta": w["beta"],
                    "start": w.get("start", "cold"), "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "flow_t_max": tau - eps,
                    "measure_every": every, "precision": main_prec},

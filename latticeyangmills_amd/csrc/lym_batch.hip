@@ -33,7 +33,8 @@ struct ConfDesc {
     long link_off;  // offset in complexes of the lattice's field in the batch buffers
     long acc_off;   // offset in reals of its RK accumulator
     long tile0;     // first tile of the lattice
-    long half0;     // first Metropolis work item of the lattice (V/2 sites per sub-sweep)
+    long item0;     // first Metropolis work item of the lattice: V/2 sites per sub-sweep,
+                    // rounded up to whole waves (64) so that no wave spans two lattices
     uint64_t seed;
     int slab;       // the lattice fits the slab staging (slab_fits)
 };
@@ -129,49 +130,66 @@ static __global__ __launch_bounds__(256) void k_batch_reduce(const ksum *__restr
     if (threadIdx.x < 4) dst[4 * c + threadIdx.x] = tot[threadIdx.x].s + tot[threadIdx.x].c;
 }
 
-// checkerboard Metropolis over the batch: work item i of sub-sweep (mu, parity)
-// is half-site i - half0[c] of lattice c.  Workgroup b takes block
-// (b % 8) * per_xcd + b / 8, so each XCD sweeps a contiguous range (as k_metropolis).
+// k_reorder_eo over the batch: thread j of plane blockIdx.y handles even-odd
+// position j - site0 of lattice c, whose V sites start at site0 = link_off / 36
+// in the batch's site numbering.
+template <typename S, bool TO_EO>
+__global__ __launch_bounds__(256) void k_batch_reorder_eo(const F2<S> *__restrict__ in, F2<S> *__restrict__ out,
+                                                          const ConfDesc *__restrict__ desc, int nconf, long nsites)
+{
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= nsites) return;
+    int lo = 0, hi = nconf - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (desc[mid].link_off <= 36 * j)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ConfDesc d = desc[lo];
+    const long jl = j - d.link_off / 36, half = d.g.V / 2;
+    const int par = jl >= half;
+    const long s = parity_site(d.g, jl - par * half, par);
+    const long base = d.link_off + (long)blockIdx.y * d.g.V;
+    if constexpr (TO_EO)
+        out[base + jl] = in[base + s];
+    else
+        out[base + s] = in[base + jl];
+}
+
+// Checkerboard Metropolis over the batch, links in even-odd order: work
+// item i of sub-sweep (mu, parity) is half-site i - item0[c] of lattice c.
+// Workgroup b takes block (b % 8) * per_xcd + b / 8, so each XCD sweeps a
+// contiguous range (as k_metropolis).  A wave never spans two lattices, so
+// the lattice is found from the wave's first item and its descriptor sits in
+// scalar registers; acceptances are added with one atomic per wave.
 template <typename S, int CAND>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_batch_metropolis(
-    F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc, int nconf, long nhalf, int mu, MetroParams p,
+    F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc, int nconf, long nitems, int mu, MetroParams p,
     unsigned per_xcd, unsigned long long *__restrict__ accepted)
 {
     const unsigned blk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
-    const long i = (long)blk * 256 + threadIdx.x;
-    int acc = 0, c = -1;
-    if (i < nhalf) {
-        int lo = 0, hi = nconf - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (desc[mid].half0 <= i)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        c = lo;
-        const ConfDesc d = desc[c];
-        MetroParams q = p;
-        q.seed = d.seed;
-        const long s = parity_site(d.g, i - d.half0, p.parity);
-        F2<S> *fc = f + d.link_off;
-        acc = metro_link_mu<S, CAND, false>(fc, d.g, mu, s, q);
+    const long iw = (long)blk * 256 + long(__builtin_amdgcn_readfirstlane(threadIdx.x & ~63u));
+    if (iw >= nitems) return;
+    int lo = 0, hi = nconf - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (desc[mid].item0 <= iw)
+            lo = mid;
+        else
+            hi = mid - 1;
     }
-    // Acceptance counts per lattice: one atomic per (wave, lattice) rather than
-    // per thread.  A wave spans consecutive work items, so it touches one
-    // lattice, or a few at a boundary: take the smallest pending lattice index,
-    // sum its lanes, repeat (the loop condition is uniform over the wave).
-    bool pending = c >= 0;
-    while (__any(pending)) {
-        int m = pending ? c : 0x7fffffff;
+    const ConfDesc d = desc[lo];
+    MetroParams q = p;
+    q.seed = d.seed;
+    const long il = iw - d.item0 + (threadIdx.x & 63);
+    int acc = 0;
+    if (il < d.g.V / 2)
+        acc = metro_link_mu<S, CAND, false>(f + d.link_off, d.g, mu, parity_site(d.g, il, p.parity), il, q);
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
-        int v = (pending && c == m) ? acc : 0;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0 && v) atomicAdd(accepted + m, (unsigned long long)v);
-        pending = pending && c != m;
-    }
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepted + lo, (unsigned long long)acc);
 }
 
 }  // namespace lym
@@ -184,7 +202,7 @@ struct lym_batch {
     std::vector<ConfDesc> desc;
     std::vector<std::array<int, 4>> dims;
     ConfDesc *d_desc = nullptr;
-    long total_links = 0, total_sites = 0, ntiles = 0, nhalf = 0;
+    long total_links = 0, total_sites = 0, ntiles = 0, nitems = 0;
     hipStream_t stream = nullptr;
     void *W[2] = {nullptr, nullptr};
     int cur = 0;
@@ -382,7 +400,7 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     b->device = device;
     b->prec = precision;
     b->nconf = nconf;
-    long links = 0, tiles = 0, half = 0, maxV = 0;
+    long links = 0, tiles = 0, items = 0, maxV = 0;
     for (int c = 0; c < nconf; c++) {
         ConfDesc d{};
         long V = 1, st = 1;
@@ -406,10 +424,10 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
         d.link_off = links * 9;
         d.acc_off = links * 8;
         d.tile0 = tiles;
-        d.half0 = half;
+        d.item0 = items;
         links += 4 * V;
         tiles += (V + lym::kSitesPerBlock - 1) / lym::kSitesPerBlock;
-        half += V / 2;
+        items += (V / 2 + 63) / 64 * 64;
         maxV = std::max(maxV, V);
         b->desc.push_back(d);
         b->dims.push_back({dims[4 * c], dims[4 * c + 1], dims[4 * c + 2], dims[4 * c + 3]});
@@ -417,7 +435,7 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     b->total_links = links;
     b->total_sites = links / 4;
     b->ntiles = tiles;
-    b->nhalf = half;
+    b->nitems = items;
     b->staging_sites = maxV;
     auto bail = [&](int code) {
         lym_batch_destroy(b);
@@ -548,17 +566,39 @@ int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds)
     return LYM_OK;
 }
 
+// Copies W[cur] into W[1 - cur] reordered to (to_eo) or from the even-odd
+// order of the Metropolis sweeps, every lattice in one launch.
+static void batch_reorder_eo(lym_batch *b, bool to_eo)
+{
+    const dim3 grid(nblocks(b->total_sites, 256), 36);
+    if (b->prec == LYM_PREC_FP64) {
+        auto *in = static_cast<const double2 *>(b->W[b->cur]);
+        auto *out = static_cast<double2 *>(b->W[1 - b->cur]);
+        if (to_eo)
+            lym::k_batch_reorder_eo<double, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+        else
+            lym::k_batch_reorder_eo<double, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+    } else {
+        auto *in = static_cast<const float2 *>(b->W[b->cur]);
+        auto *out = static_cast<float2 *>(b->W[1 - b->cur]);
+        if (to_eo)
+            lym::k_batch_reorder_eo<float, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+        else
+            lym::k_batch_reorder_eo<float, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+    }
+}
+
 extern "C++" template <typename S>
 static void launch_batch_metropolis(lym_batch *b, unsigned nb, int mu, const lym::MetroParams &p)
 {
     const unsigned per_xcd = (nb + 7) / 8;
     auto *f = static_cast<lym::F2<S> *>(b->W[b->cur]);
     if (p.candidate == LYM_CAND_SU2)
-        lym::k_batch_metropolis<S, 0><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
+        lym::k_batch_metropolis<S, 0><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nitems, mu, p, per_xcd, b->accepted);
     else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max)
-        lym::k_batch_metropolis<S, 1><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
+        lym::k_batch_metropolis<S, 1><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nitems, mu, p, per_xcd, b->accepted);
     else  // every candidate has c1 < 8 eps^2: series-only exponential
-        lym::k_batch_metropolis<S, 2><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nhalf, mu, p, per_xcd, b->accepted);
+        lym::k_batch_metropolis<S, 2><<<8 * per_xcd, 256, 0, b->stream>>>(f, b->d_desc, b->nconf, b->nitems, mu, p, per_xcd, b->accepted);
 }
 
 int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,
@@ -576,7 +616,13 @@ int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int ns
     BHIP(b, hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(ConfDesc) * b->nconf, hipMemcpyHostToDevice, b->stream));
     lym::k_zero_u64<<<nblocks(b->nconf, 64), 64, 0, b->stream>>>(b->accepted, b->nconf);
     const int nu = candidate == LYM_CAND_SU2 ? 13 : 9;
-    const unsigned nb = nblocks(b->nhalf, 256);
+    // the sweeps run on an even-odd copy in the other buffer, reordered back at the end
+    if (nsweeps > 0) {
+        batch_reorder_eo(b, true);
+        BLAUNCHED(b, "k_batch_reorder_eo");
+        b->cur ^= 1;
+    }
+    const unsigned nb = nblocks(b->nitems, 256);
     for (int sw = 0; sw < nsweeps; sw++) {
         lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, 0, nullptr};
         for (int mu = 0; mu < 4; mu++)
@@ -588,6 +634,11 @@ int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int ns
                     launch_batch_metropolis<float>(b, nb, mu, p);
                 BLAUNCHED(b, "k_batch_metropolis");
             }
+    }
+    if (nsweeps > 0) {
+        batch_reorder_eo(b, false);
+        BLAUNCHED(b, "k_batch_reorder_eo");
+        b->cur ^= 1;
     }
     std::vector<unsigned long long> acc(b->nconf);
     BHIP(b, hipMemcpyAsync(acc.data(), b->accepted, sizeof(unsigned long long) * b->nconf, hipMemcpyDeviceToHost,

@@ -424,6 +424,28 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
     return LYM_OK;
 }
 
+// Copies W[cur] into W[1 - cur] reordered to (TO_EO) or from the even-odd
+// order of the Metropolis sweeps.
+static void reorder_eo(lym_ctx *c, bool to_eo)
+{
+    const dim3 grid(blocks_for(c->g.V, 256), 36);
+    if (c->prec == LYM_PREC_FP64) {
+        auto *in = static_cast<const double2 *>(c->W[c->cur]);
+        auto *out = static_cast<double2 *>(c->W[1 - c->cur]);
+        if (to_eo)
+            lym::k_reorder_eo<double, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+        else
+            lym::k_reorder_eo<double, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+    } else {
+        auto *in = static_cast<const float2 *>(c->W[c->cur]);
+        auto *out = static_cast<float2 *>(c->W[1 - c->cur]);
+        if (to_eo)
+            lym::k_reorder_eo<float, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+        else
+            lym::k_reorder_eo<float, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+    }
+}
+
 // One Metropolis sub-sweep launch; the candidate kind and the test mode pick
 // the kernel instantiation, so each carries only its own registers.
 extern "C++" template <typename S>
@@ -475,6 +497,13 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     }
     lym::k_zero_u64<<<1, 64, 0, c->stream>>>(c->accepted, 1);
     LYM_LAUNCHED(c, "k_zero_u64");
+    // The sweeps run on an even-odd copy of the links in the other buffer,
+    // which is reordered back at the end.
+    if (nsweeps > 0) {
+        reorder_eo(c, true);
+        LYM_LAUNCHED(c, "k_reorder_eo");
+        c->cur ^= 1;
+    }
     const unsigned nb = blocks_for(V / 2, 256);
     for (int sw = 0; sw < nsweeps; sw++) {
         lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, seed,
@@ -488,6 +517,11 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
                     launch_metropolis<float>(c, nb, mu, p);
                 LYM_LAUNCHED(c, "k_metropolis");
             }
+    }
+    if (nsweeps > 0) {
+        reorder_eo(c, false);
+        LYM_LAUNCHED(c, "k_reorder_eo");
+        c->cur ^= 1;
     }
     unsigned long long acc = 0;
     LYM_HIP(c, hipMemcpyAsync(&acc, c->accepted, sizeof acc, hipMemcpyDeviceToHost, c->stream));

@@ -30,15 +30,17 @@ template <typename S> using F2 = vec2_t<S>;
 // ---------------------------------------------------------- link sources ---
 
 // A link read of the staple is tagged at compile time with its offset from
-// the site being updated: DX, DY in {-1, 0, 1} along x and y, and ZT = 1 when
-// the offset has a z or t part.  A link source decides from the tag alone
-// whether the read comes from LDS, so no lane ever branches on it.
+// the site being updated: DX, DY in {-1, 0, 1} along x and y, ZT = 1 when
+// the offset has a z or t part, and FLIP = 1 when the offset is one lattice
+// step (the other checkerboard parity) rather than zero or two.  A link
+// source decides from the tag alone whether the read comes from LDS, so no
+// lane ever branches on it.
 
 // Links of a field read straight from HBM (through L1/L2).
 template <typename S> struct GlobalLinks {
     const F2<S> *f;
     long V;
-    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
+    template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
         return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
     }
@@ -66,7 +68,7 @@ template <typename S, bool XROWS> struct RunLinks {
     long V;
     long site0;
     const F2<S> *lds;
-    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
+    template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
         if constexpr (ZT == 0 && DY == 0 && (DX == 0 || XROWS))
             return lds_link<R, S>(lds + dir * 9 * kSitesPerBlock + unsigned(site - site0), kSitesPerBlock);
@@ -94,7 +96,7 @@ template <typename S> struct SlabLinks {
     long V;
     const F2<S> *slab;
     int lx_mask, row, x;  // Lx - 1; this lane's row in the run and x coordinate
-    template <typename R, int DX, int DY, int ZT> LYM_DI su3<R> get(int dir, long site) const
+    template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
         if constexpr (ZT == 0) {
             const int li = (row + 1 + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
@@ -102,6 +104,24 @@ template <typename S> struct SlabLinks {
         } else {
             return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
         }
+    }
+};
+
+// Links in the even-odd order of the Metropolis sweeps: the link of site s,
+// of parity P(s) = (x + y + z + t) & 1, sits at P(s) V/2 + s/2.  With Lx even,
+// s/2 is the rank of s among the sites of its parity in lexicographic order,
+// so a wave that updates 64 consecutive sites of one parity reads 64
+// consecutive complexes per matrix element, for its own links and for every
+// neighbour; in lexicographic order it would use every second one.
+template <typename S> struct EOLinks {
+    const F2<S> *f;
+    long V;
+    unsigned half;  // V / 2
+    unsigned par;   // parity of the site being updated
+    template <typename R, int DX, int DY, int ZT, int FLIP> LYM_DI su3<R> get(int dir, long site) const
+    {
+        const unsigned base = (par ^ unsigned(FLIP)) ? half : 0u;
+        return load_link<R, S>(f + (long)dir * 9 * V, V, base + (unsigned(site) >> 1));
     }
 };
 
@@ -122,37 +142,37 @@ LYM_DI void staple_plane(const Src &src, long s, const Nbr &n, su3<R> &om)
     const long y = s + n.bwd[NU];  // x - NU
     if constexpr (sizeof(R) == 4) {
         {
-            su3<R> a = src.template get<R, MX, MY, MZ>(NU, s + n.fwd[MU]);
-            su3<R> b = src.template get<R, NX, NY, NZ>(MU, s + n.fwd[NU]);
+            su3<R> a = src.template get<R, MX, MY, MZ, 1>(NU, s + n.fwd[MU]);
+            su3<R> b = src.template get<R, NX, NY, NZ, 1>(MU, s + n.fwd[NU]);
             su3<R> t, u;
             mul_na_rows01(t, a, b);
-            su3<R> c = src.template get<R, 0, 0, 0>(NU, s);
+            su3<R> c = src.template get<R, 0, 0, 0, 0>(NU, s);
             mul_na_rows01(u, t, c);
             acc_su3_rows01(om, u);
         }
         {
-            su3<R> a = src.template get<R, -NX, -NY, NZ>(MU, y);
-            su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ>(NU, y + n.fwd[MU]);
+            su3<R> a = src.template get<R, -NX, -NY, NZ, 1>(MU, y);
+            su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ, 0>(NU, y + n.fwd[MU]);
             su3<R> t, u;
             mul_nn_cols01(t, a, b);  // (a b)^+ rows 0, 1 need columns 0, 1 of a b
-            su3<R> c = src.template get<R, -NX, -NY, NZ>(NU, y);
+            su3<R> c = src.template get<R, -NX, -NY, NZ, 1>(NU, y);
             mul_an_rows01(u, t, c);
             acc_su3_rows01(om, u);
         }
         return;
     }
     {
-        su3<R> a = src.template get<R, MX, MY, MZ>(NU, s + n.fwd[MU]);
-        su3<R> b = src.template get<R, NX, NY, NZ>(MU, s + n.fwd[NU]);
+        su3<R> a = src.template get<R, MX, MY, MZ, 1>(NU, s + n.fwd[MU]);
+        su3<R> b = src.template get<R, NX, NY, NZ, 1>(MU, s + n.fwd[NU]);
         su3<R> t = mul_na(a, b);
-        su3<R> c = src.template get<R, 0, 0, 0>(NU, s);
+        su3<R> c = src.template get<R, 0, 0, 0, 0>(NU, s);
         acc_na(om, t, c);
     }
     {
-        su3<R> a = src.template get<R, -NX, -NY, NZ>(MU, y);
-        su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ>(NU, y + n.fwd[MU]);
+        su3<R> a = src.template get<R, -NX, -NY, NZ, 1>(MU, y);
+        su3<R> b = src.template get<R, MX - NX, MY - NY, MZ | NZ, 0>(NU, y + n.fwd[MU]);
         su3<R> t = mul_nn(a, b);
-        su3<R> c = src.template get<R, -NX, -NY, NZ>(NU, y);
+        su3<R> c = src.template get<R, -NX, -NY, NZ, 1>(NU, y);
         acc_an(om, t, c);
     }
 }
@@ -782,21 +802,24 @@ struct MetroParams {
 // Metropolis update of link (mu, s): nhit hits of U' = X U with
 // dS = -(beta/3) Re Tr[(U' - U) Omega], accepted iff exp(-dS) > u
 // (src/Apps/gaugefieldfactory.cpp:205-220, src/Actions/puregauge.cpp:28).
+// The field f is in even-odd order (EOLinks); s is the lexicographic index
+// (it keys the random numbers) and i its rank within its parity.
 // Arithmetic in FP64; FP32 storage is reunitarised before the store.
 // CAND (0: R S T product, 13 uniforms per hit; 1: exp(i eps H), 9; 2: the
 // same with 8 eps^2 <= kSeriesC1Max, series-only exponential) and
 // PREDRAWN (uniforms read from an array instead of drawn from Philox) are
 // compile-time, so each kernel carries the registers of one candidate only.
 template <typename S, int MU, int CAND, bool PREDRAWN>
-LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroParams &p)
+LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, long i, const MetroParams &p)
 {
     constexpr int NU = (CAND == 0) ? 13 : 9;
     constexpr int BLOCKS = (NU + 3) / 4;  // Philox outputs 4 uniforms per call
     const long V = g.V;
+    const unsigned half = unsigned(V / 2), own = unsigned(p.parity) * half + unsigned(i);
     Nbr n = neighbours(g, s);
-    su3<double> om = staple<double, S, MU>(f, V, s, n);
+    su3<double> om = staple_from<double, MU>(EOLinks<S>{f, V, half, unsigned(p.parity)}, s, n);
     F2<S> *Fm = f + (long)MU * 9 * V;
-    su3<double> U = load_link<double, S>(Fm, V, s);
+    su3<double> U = load_link<double, S>(Fm, V, own);
     const uint32_t lk = uint32_t(MU * V + s);
     const uint2 key = make_uint2(uint32_t(p.seed), uint32_t(p.seed >> 32));
     int acc = 0;
@@ -832,19 +855,19 @@ LYM_DI int metro_link(F2<S> *__restrict__ f, const Geom g, long s, const MetroPa
         }
     }
     if constexpr (sizeof(S) == 4) reunitarize(U);
-    store_link<S, double>(Fm, V, s, U);
+    store_link<S, double>(Fm, V, own, U);
     return acc;
 }
 
 // metro_link for a direction known only at run time (uniform over the wave)
 template <typename S, int CAND, bool PREDRAWN>
-LYM_DI int metro_link_mu(F2<S> *__restrict__ f, const Geom g, int mu, long s, const MetroParams &p)
+LYM_DI int metro_link_mu(F2<S> *__restrict__ f, const Geom g, int mu, long s, long i, const MetroParams &p)
 {
     switch (mu) {
-    case 0: return metro_link<S, 0, CAND, PREDRAWN>(f, g, s, p);
-    case 1: return metro_link<S, 1, CAND, PREDRAWN>(f, g, s, p);
-    case 2: return metro_link<S, 2, CAND, PREDRAWN>(f, g, s, p);
-    default: return metro_link<S, 3, CAND, PREDRAWN>(f, g, s, p);
+    case 0: return metro_link<S, 0, CAND, PREDRAWN>(f, g, s, i, p);
+    case 1: return metro_link<S, 1, CAND, PREDRAWN>(f, g, s, i, p);
+    case 2: return metro_link<S, 2, CAND, PREDRAWN>(f, g, s, i, p);
+    default: return metro_link<S, 3, CAND, PREDRAWN>(f, g, s, i, p);
     }
 }
 
@@ -860,10 +883,29 @@ LYM_DI long parity_site(const Geom g, long i, int parity)
     return long(x + uint32_t(g.L[0]) * r);
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; workgroup b works on
-// block (b % 8) * per_xcd + b / 8, so each XCD sweeps one contiguous range
-// of the lattice and the neighbour links along y and z stay in its L2.  The
-// grid is 8 * per_xcd workgroups; blocks past the lattice return at once.
+// Reorders the 36 (direction, element) planes of a link field between the
+// lexicographic order and the even-odd order of EOLinks; one thread per
+// complex, thread j of a plane handles even-odd position j.
+template <typename S, bool TO_EO>
+__global__ __launch_bounds__(256) void k_reorder_eo(const F2<S> *__restrict__ in, F2<S> *__restrict__ out, Geom g)
+{
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= g.V) return;
+    const long half = g.V / 2, base = (long)blockIdx.y * g.V;
+    const int par = j >= half;
+    const long s = parity_site(g, j - par * half, par);
+    if constexpr (TO_EO)
+        out[base + j] = in[base + s];
+    else
+        out[base + s] = in[base + j];
+}
+
+// One Metropolis sub-sweep (direction mu, one parity) over a field in
+// even-odd order.  Workgroups are dispatched round-robin over the 8 XCDs;
+// workgroup b works on block (b % 8) * per_xcd + b / 8, so each XCD sweeps
+// one contiguous range of the lattice and the neighbour links along y and z
+// stay in its L2.  The grid is 8 * per_xcd workgroups; blocks past the
+// lattice return at once.
 template <typename S, int CAND, bool PREDRAWN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(
     F2<S> *__restrict__ f, Geom g, int mu, MetroParams p, unsigned per_xcd, unsigned long long *__restrict__ accepted)
@@ -871,7 +913,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ?
     const unsigned blk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
     const long i = (long)blk * 256 + threadIdx.x;
     int acc = 0;
-    if (i < g.V / 2) acc = metro_link_mu<S, CAND, PREDRAWN>(f, g, mu, parity_site(g, i, p.parity), p);
+    if (i < g.V / 2) acc = metro_link_mu<S, CAND, PREDRAWN>(f, g, mu, parity_site(g, i, p.parity), i, p);
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) acc += __shfl_xor(acc, m, 64);
     if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepted, (unsigned long long)acc);

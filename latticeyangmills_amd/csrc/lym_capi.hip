@@ -113,6 +113,8 @@ void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *
         lym::k_flow_stage<S, STAGE, reunit, 3><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
     else if (lym::kSitesPerBlock % c->dims[0] == 0)  // runs hold whole x-rows
         lym::k_flow_stage<S, STAGE, reunit, 2><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+    else if (lym::tile_fits(c->dims))  // 16 x 4 blocks that tile every (z, t) plane
+        lym::k_flow_stage<S, STAGE, reunit, 5><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
     else
         lym::k_flow_stage<S, STAGE, reunit, 1><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
 }

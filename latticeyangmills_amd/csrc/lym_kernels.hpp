@@ -125,6 +125,30 @@ template <typename S> struct EOLinks {
     }
 };
 
+// Tile staging for lattices whose x-rows do not divide a run (Lx = 48, 80,
+// ...): the 64 sites of a workgroup are a kTileX x kTileY = 16 x 4 block of
+// one (z, t) plane, and the block with a one-site halo ring in x and y
+// (periodic) is staged as tile[(dir*9 + k)*kSlabSites + (ly + 1)*(kTileX + 2)
+// + lx + 1], ly = -1 .. 4, lx = -1 .. 16: 108 sites.  Every read whose offset
+// has no z or t part comes from LDS, as with the slab.  A tile row of one
+// element is a whole 128-byte line in FP32; narrower tiles (8 x 8 for
+// Lx = 24) were measured slower than staging own links only.
+constexpr int kTileX = 16, kTileY = kSitesPerBlock / kTileX;
+inline bool tile_fits(const int dims[4]) { return dims[0] % kTileX == 0 && dims[1] % kTileY == 0; }
+template <typename S> struct TileLinks {
+    const F2<S> *f;
+    long V;
+    const F2<S> *tile;
+    int self;  // this lane's slot (ly + 1) * (kTileX + 2) + lx + 1
+    template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
+    {
+        if constexpr (ZT == 0)
+            return lds_link<R, S>(tile + dir * 9 * kSlabSites + self + DY * (kTileX + 2) + DX, kSlabSites);
+        else
+            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+    }
+};
+
 // ---------------------------------------------------------------- staple ---
 
 // The two staples of plane (MU, NU) for link U_MU(x):
@@ -246,6 +270,8 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
 //   2  the same, runs holding whole x-rows (reads along x too)
 //   3  the run's rows plus one halo row on each side in y (SlabLinks, every
 //      read without a z or t part)
+//   5  run r is tile r instead (TileLinks: a 16 x 4 block of one (z, t)
+//      plane plus a halo ring in x and y; every read without a z or t part)
 // All 256 threads reach the barrier, then lanes past the end of the lattice
 // drop out.  The staging buffer is free again when the function returns
 // (trailing barrier), so a persistent caller may loop.
@@ -255,6 +281,72 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
 {
     const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
     const int lane = threadIdx.x & 63;
+    if constexpr (LDS == 5) {
+        // Tiles cover the lattice exactly (checked on the host), so every lane
+        // is a site.  Tile r = tx + ntx (ty + nty zt), zt = z + Lz t: the same
+        // (z, t) plane as run r, so the run schedules keep their locality.
+        // Wave mu stages direction mu: lane l its own link, and lanes below
+        // 2 (TX + 2) + 2 TY = 44 one halo site each (rows -1 and TY, then
+        // columns -1 and TX).
+        constexpr int TX = kTileX, TY = kTileY, W = TX + 2;
+        __shared__ F2<S> tile[4 * 9 * kSlabSites];
+        const int Lx = g.L[0], Ly = g.L[1];
+        const uint32_t r = uint32_t(site0 / kSitesPerBlock), ntx = uint32_t(Lx / TX), nty = uint32_t(Ly / TY);
+        const uint32_t q = r / ntx, tx = r - q * ntx, zt = q / nty, ty = q - zt * nty;
+        const int x0 = int(tx) * TX, y0 = int(ty) * TY;
+        const long pb = (long)zt * Lx * Ly;
+        const int lx = lane % TX, ly = lane / TX;
+        const long s = pb + x0 + lx + (long)Lx * (y0 + ly);
+        const F2<S> *plane = win + (long)mu * 9 * g.V;
+        F2<S> *dst = tile + mu * 9 * kSlabSites;
+        const int self = (ly + 1) * W + lx + 1;
+        {
+            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(s));
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                dst[k * kSlabSites + self].x = u.e[k].re;
+                dst[k * kSlabSites + self].y = u.e[k].im;
+            }
+        }
+        if (lane < 2 * W + 2 * TY) {
+            int hx, hy;
+            if (lane < W) {
+                hx = lane - 1;
+                hy = -1;
+            } else if (lane < 2 * W) {
+                hx = lane - W - 1;
+                hy = TY;
+            } else if (lane < 2 * W + TY) {
+                hx = -1;
+                hy = lane - 2 * W;
+            } else {
+                hx = TX;
+                hy = lane - 2 * W - TY;
+            }
+            int gx = x0 + hx, gy = y0 + hy;
+            gx += (gx < 0) ? Lx : 0;
+            gx -= (gx >= Lx) ? Lx : 0;
+            gy += (gy < 0) ? Ly : 0;
+            gy -= (gy >= Ly) ? Ly : 0;
+            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(pb + gx + (long)Lx * gy));
+            const int li = (hy + 1) * W + hx + 1;
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                dst[k * kSlabSites + li].x = u.e[k].re;
+                dst[k * kSlabSites + li].y = u.e[k].im;
+            }
+        }
+        __syncthreads();
+        const TileLinks<S> src{win, g.V, tile, self};
+        switch (mu) {
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        }
+        __syncthreads();
+        return;
+    }
     const long s = site0 + lane;
     const bool active = s < g.V;
     if constexpr (LDS == 3) {

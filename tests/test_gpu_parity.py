@@ -272,3 +272,24 @@ def test_slab_staging(dims, kind):
         ctx.upload(links)
         ctx.flow_steps(0.02, 1)
         assert np.max(np.abs(ctx.download() - want)) < 2e-6
+
+
+@pytest.mark.parametrize("dims", [(48, 4, 2, 2), (48, 8, 2, 2), (48, 12, 2, 4), (80, 4, 2, 2), (112, 8, 2, 2)])
+@pytest.mark.parametrize("kind", ["linear", "column"])
+def test_tile_staging(dims, kind):
+    """Lattices with Lx a multiple of 16 that does not divide 64 take 16 x 4
+    tiles of one (z, t) plane with a halo ring in x and y (Ly = 4 makes both
+    halo rows the tile's own rows): one FP64 step equals the oracle, and the
+    mixed path stays within FP32 rounding of it."""
+    links = orc.hot_start(dims, 43)
+    want = orc.flow_step(links, dims, 0.02)
+    with LatticeContext(dims) as ctx:
+        ctx.set_schedule(kind, 1024 if kind == "column" else 0, True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        assert np.max(np.abs(ctx.download() - want)) < 1e-12
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.set_schedule(kind, 1024 if kind == "column" else 0, True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        assert np.max(np.abs(ctx.download() - want)) < 2e-6

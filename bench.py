@@ -44,7 +44,7 @@ FLOP_PER_LINK_UPDATE = {"mixed": 8460, "fp64": 9620}
 # scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
 # back); the run, its commands and the per-stage figures are in PROFILE_FILE.
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
-PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.361e9, ("mixed-32x64", "fp64"): 7.043e9}
+PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.359e9, ("mixed-32x64", "fp64"): 5.445e9}
 
 WORKLOADS = {
     # 1000 sweeps from a Haar-random start put t0 (t^2 <E> = 0.3) near t = 5.1,

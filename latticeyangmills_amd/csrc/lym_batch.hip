@@ -58,13 +58,14 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
 // kept live across tiles); true: a persistent grid strides over the slots.
 // LDS as flow_block, and 4: per lattice, slab staging where the lattice fits
 // it and the run's own links elsewhere (the lattice of a tile, and so the
-// choice, is uniform over the workgroup).
+// choice, is uniform over the workgroup); both modes stage into one array.
 template <typename S, int STAGE, bool REUNIT, int LDS, bool PERSIST>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
                                                           int nconf, long ntiles, Sched sc, double eps)
 {
     const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
+    __shared__ F2<S> lds[stage_lds_elems<LDS == 4 ? 3 : LDS>()];
     for (long k = blockIdx.x; k < slots; k += gridDim.x) {
         const long tile = sched_run(sc, unsigned(k));  // uniform over the workgroup
         if (tile >= 0) {
@@ -73,11 +74,14 @@ __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restric
             const long site0 = (tile - d.tile0) * kSitesPerBlock;
             if constexpr (LDS == 4) {
                 if (d.slab)
-                    flow_block<S, STAGE, REUNIT, 3>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+                    flow_block<S, STAGE, REUNIT, 3>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
+                                                    lds);
                 else
-                    flow_block<S, STAGE, REUNIT, 1>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+                    flow_block<S, STAGE, REUNIT, 1>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
+                                                    lds);
             } else {
-                flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps);
+                flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
+                                                  lds);
             }
         }
         if constexpr (!PERSIST) break;

@@ -273,11 +273,17 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
 //   5  run r is tile r instead (TileLinks: a 16 x 4 block of one (z, t)
 //      plane plus a halo ring in x and y; every read without a z or t part)
 // All 256 threads reach the barrier, then lanes past the end of the lattice
-// drop out.  The staging buffer is free again when the function returns
-// (trailing barrier), so a persistent caller may loop.
+// drop out.  The staging buffer lds (the kernel's LDS array, at least
+// stage_lds_elems<LDS>() complexes) is free again when the function returns
+// (trailing barrier), so a persistent caller may loop, and a kernel that
+// picks the mode per workgroup passes every mode the same array.
+template <int LDS> constexpr int stage_lds_elems()
+{
+    return LDS == 0 ? 1 : (LDS == 3 || LDS == 5) ? 4 * 9 * kSlabSites : 4 * 9 * kSitesPerBlock;
+}
 template <typename S, int STAGE, bool REUNIT, int LDS>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
-                       const Geom g, long site0, double eps)
+                       const Geom g, long site0, double eps, F2<S> *__restrict__ lds)
 {
     const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
     const int lane = threadIdx.x & 63;
@@ -289,7 +295,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         // 2 (TX + 2) + 2 TY = 44 one halo site each (rows -1 and TY, then
         // columns -1 and TX).
         constexpr int TX = kTileX, TY = kTileY, W = TX + 2;
-        __shared__ F2<S> tile[4 * 9 * kSlabSites];
+        F2<S> *const tile = lds;
         const int Lx = g.L[0], Ly = g.L[1];
         const uint32_t r = uint32_t(site0 / kSitesPerBlock), ntx = uint32_t(Lx / TX), nty = uint32_t(Ly / TY);
         const uint32_t q = r / ntx, tx = r - q * ntx, zt = q / nty, ty = q - zt * nty;
@@ -354,7 +360,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         // is a lattice site (checked on the host).  Wave mu stages direction mu:
         // the run's rows at slab rows 1..R, and the rows y0 - 1 and y0 + R
         // (periodic in y) at slab rows 0 and R + 1.
-        __shared__ F2<S> slab[4 * 9 * kSlabSites];
+        F2<S> *const slab = lds;
         const int Lx = g.L[0], Ly = g.L[1], R = kSitesPerBlock / Lx;
         const int y0 = int(uint32_t(site0) / uint32_t(Lx) % uint32_t(Ly));
         const long lo = (y0 > 0) ? site0 - Lx : site0 - Lx + long(Ly) * Lx;
@@ -390,7 +396,6 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         }
         __syncthreads();
     } else if constexpr (LDS != 0) {
-        __shared__ F2<S> lds[4 * 9 * kSitesPerBlock];
         if (active) {
             const su3<S> u = load_link<S, S>(win + (long)mu * 9 * g.V, g.V, unsigned(s));
 #pragma unroll
@@ -430,7 +435,8 @@ __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ wi
 {
     const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
     if (run < 0) return;
-    flow_block<S, STAGE, REUNIT, LDS>(win, wout, acc, g, run * kSitesPerBlock, eps);
+    __shared__ F2<S> lds[stage_lds_elems<LDS>()];
+    flow_block<S, STAGE, REUNIT, LDS>(win, wout, acc, g, run * kSitesPerBlock, eps, lds);
 }
 
 // ----------------------------------------------------------- observables ---

@@ -94,7 +94,10 @@ int lym_flow(lym_ctx *ctx, double eps, double tau, int measure_every, double *se
  * NULL: Philox-4x32-10 uniforms, counter {mu*V+s, sweep, hit, k/4}, key seed.
  * predrawn != NULL (test mode): uniform k of hit h of link l in sweep w is
  * predrawn[((w*4V + l)*nhit + h)*n + k], n = 13 (LYM_CAND_SU2) or 9 (LYM_CAND_EXPH).
- * Extents must be even.  *accepted = number of accepted hits. */
+ * Extents must be even.  *accepted = number of accepted hits.  The sweeps
+ * run on a copy of the links in even-odd site order held in the context's
+ * second link buffer; the updated links are back in the usual order when the
+ * call returns (it synchronises the context's stream). */
 int lym_metropolis(lym_ctx *ctx, double beta, double eps, int nhit, int nsweeps, int first_sweep,
                    int candidate, uint64_t seed, const double *predrawn, int64_t *accepted);
 

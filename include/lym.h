@@ -86,6 +86,16 @@ int lym_flow_steps(lym_ctx *ctx, double eps, int nsteps);
  * doubles; *rows_out receives rows. */
 int lym_flow(lym_ctx *ctx, double eps, double tau, int measure_every, double *series, int max_rows,
              int *rows_out);
+/* No counterpart in the reference (its flow loop, src/Apps/wilsonflow.cpp:115-136,
+ * calls each step directly).  on != 0: lym_flow records its launches (steps
+ * and measurements) once as a HIP graph and replays it while eps, rows,
+ * measure_every and the start buffer repeat; one graph is kept per start
+ * buffer, and a change of schedule drops both.  Results are those of the
+ * launches it replays.  It does not make the flow faster on MI355X: even on
+ * 8^4 each stage kernel runs ~10 us, which hides the asynchronous launches
+ * (99 steps: 3.05 ms eager, 2.98 ms replayed).  It serves callers that want
+ * one replayable unit per flow.  Default off. */
+int lym_set_graph(lym_ctx *ctx, int on);
 
 /* GaugeFieldFactory::MCUpdate / updateLink (src/Apps/gaugefieldfactory.cpp:192-220)
  * with the Delta S of PureGauge::compute (src/Actions/puregauge.cpp:28), as a

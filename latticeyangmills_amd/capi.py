@@ -47,6 +47,7 @@ def _sig(lib):
     lib.lym_observables.argtypes = [_P, _D]
     lib.lym_flow_steps.argtypes = [_P, c.c_double, _I]
     lib.lym_flow.argtypes = [_P, c.c_double, c.c_double, _I, _D, _I, c.POINTER(_I)]
+    lib.lym_set_graph.argtypes = [_P, _I]
     lib.lym_metropolis.argtypes = [_P, c.c_double, c.c_double, _I, _I, _I, _I, c.c_uint64, _D,
                                    c.POINTER(c.c_int64)]
     lib.lym_synchronize.argtypes = [_P]
@@ -168,6 +169,10 @@ class LatticeContext:
         self._check(self._lib.lym_flow(self._h, eps, tau, measure_every, _dptr(series), series.shape[0],
                                        ctypes.byref(r)), "lym_flow")
         return series[: r.value]
+
+    def set_graph(self, on):
+        """flow() records its launches once as a HIP graph and replays it."""
+        self._check(self._lib.lym_set_graph(self._h, 1 if on else 0), "lym_set_graph")
 
     def synchronize(self):
         self._check(self._lib.lym_synchronize(self._h), "lym_synchronize")

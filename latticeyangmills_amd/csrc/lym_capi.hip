@@ -38,6 +38,12 @@ struct lym_ctx {
     std::vector<hipEvent_t> events;
     lym::HostSchedule sched;  // work schedule of the stencil kernels
     int *d_order = nullptr;
+    // lym_flow as a captured graph (lym_set_graph), one per start buffer
+    // W[cur]; a graph holds the buffer pointers of its capture, so every
+    // reallocation of d_order, partials or obs drops both
+    bool use_graph = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    std::string graph_key[2];
     std::string err;
 };
 
@@ -71,6 +77,15 @@ size_t real_size(const lym_ctx *c) { return c->prec == LYM_PREC_FP64 ? sizeof(do
 
 unsigned blocks_for(long n, int per) { return unsigned((n + per - 1) / per); }
 
+void drop_graphs(lym_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+        c->graph[k] = nullptr;
+        c->graph_key[k].clear();
+    }
+}
+
 lym::Sched dev_sched(const lym_ctx *c) { return lym::Sched{c->sched.order.empty() ? nullptr : c->d_order, c->sched.per_xcd}; }
 
 // workgroups of a stencil launch under the current schedule
@@ -83,6 +98,7 @@ int apply_schedule(lym_ctx *c, int kind, int target)
 {
     std::vector<std::array<int, 4>> d{{c->dims[0], c->dims[1], c->dims[2], c->dims[3]}};
     lym::HostSchedule h = lym::build_schedule(d, {0L}, kind, target);
+    drop_graphs(c);
     if (c->d_order) (void)hipFree(c->d_order);
     c->d_order = nullptr;
     if (!h.order.empty()) {
@@ -164,6 +180,7 @@ int ensure_slots(lym_ctx *c, int n)
 {
     if (n <= c->obs_cap) return LYM_OK;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
+    drop_graphs(c);
     if (c->obs) (void)hipFree(c->obs);
     c->obs = nullptr;
     LYM_HIP(c, hipMalloc(&c->obs, sizeof(double) * 4 * size_t(n)));
@@ -251,6 +268,7 @@ int lym_destroy(lym_ctx *c)
     if (!c) return LYM_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
                     (void *)c->accepted, (void *)c->check, (void *)c->predrawn, (void *)c->d_order})
         if (p) (void)hipFree(p);
@@ -397,18 +415,48 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
     const int nmeas = 1 + (rows - 1) / measure_every;
     int rc = ensure_slots(c, nmeas);
     if (rc) return rc;
-    int slot = 0;
-    if ((rc = observables(c, slot++))) return rc;
-    for (int t = 1; t < rows; t++) {
-        if ((rc = step(c, eps))) return rc;
-        if (t % measure_every == 0)
-            if ((rc = observables(c, slot++))) return rc;
+    auto enqueue = [&]() -> int {
+        int slot = 0, r = observables(c, slot++);
+        for (int t = 1; t < rows && !r; t++) {
+            r = step(c, eps);
+            if (!r && t % measure_every == 0) r = observables(c, slot++);
+        }
+        return r;
+    };
+    if (c->use_graph) {
+        // key: the flow parameters (the buffers are fixed while the graph lives)
+        char key[96];
+        snprintf(key, sizeof key, "%a|%d|%d", eps, rows, measure_every);
+        const int k = c->cur;
+        if (!c->graph[k] || c->graph_key[k] != key) {
+            if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+            c->graph[k] = nullptr;
+            c->graph_key[k].clear();
+            LYM_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            c->cur = k;  // capture enqueued nothing: the field is where it was
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+            const hipError_t ei = hipGraphInstantiate(&c->graph[k], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) return hipfail(c, ei, "hipGraphInstantiate");
+            c->graph_key[k] = key;
+        }
+        LYM_HIP(c, hipGraphLaunch(c->graph[k], c->stream));
+        if ((rows - 1) & 1) c->cur = 1 - k;
+    } else if ((rc = enqueue())) {
+        return rc;
     }
     std::vector<double> raw(4 * size_t(nmeas));
     LYM_HIP(c, hipMemcpyAsync(raw.data(), c->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, c->stream));
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     double tcur = 0.0;
-    slot = 0;
+    int slot = 0;
     for (int t = 0; t < rows; t++) {
         if (t > 0) tcur += eps;  // src/Apps/wilsonflow.cpp:126
         double *row = series + 4 * size_t(t);
@@ -423,6 +471,16 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             row[1] = row[2] = row[3] = NAN;
         }
     }
+    return LYM_OK;
+}
+
+int lym_set_graph(lym_ctx *c, int on)
+{
+    if (!c) return LYM_ERR_ARG;
+    LYM_HIP(c, hipSetDevice(c->device));
+    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    c->use_graph = on != 0;
+    if (!c->use_graph) drop_graphs(c);
     return LYM_OK;
 }
 

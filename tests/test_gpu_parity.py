@@ -293,3 +293,34 @@ def test_tile_staging(dims, kind):
         ctx.upload(links)
         ctx.flow_steps(0.02, 1)
         assert np.max(np.abs(ctx.download() - want)) < 2e-6
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flow_graph_replays_match_eager(precision):
+    """lym_flow replayed from its captured graph equals the eager launches bit
+    for bit, on every replay: three replays with eager work and host
+    synchronisation between them; an odd step count alternates the start
+    buffer, so both cached graphs are captured and replayed; a schedule
+    change drops them and a new capture follows."""
+    dims = (4, 4, 4, 8)
+    links = orc.hot_start(dims, 47)
+    eps, tau, every = 0.02, 0.14, 2  # 7 rows, 6 steps
+    with LatticeContext(dims, precision) as a, LatticeContext(dims, precision) as b:
+        b.set_graph(True)
+        a.upload(links)
+        b.upload(links)
+        for rep in range(4):
+            if rep == 2:
+                a.set_schedule("column", 1024, True)
+                b.set_schedule("column", 1024, True)
+            want = a.flow(eps, tau, every)
+            got = b.flow(eps, tau, every)
+            assert np.array_equal(np.isnan(want), np.isnan(got))
+            assert np.array_equal(want[~np.isnan(want)], got[~np.isnan(got)]), rep
+            np.testing.assert_array_equal(b.download(), a.download())
+            a.flow_steps(eps, 1)  # eager work between replays: odd, so the start buffer alternates
+            b.flow_steps(eps, 1)
+            b.synchronize()
+            a.synchronize()
+        b.set_graph(False)
+        np.testing.assert_array_equal(b.flow(eps, tau, every)[:, 1:3], a.flow(eps, tau, every)[:, 1:3])

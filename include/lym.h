@@ -104,10 +104,12 @@ int lym_set_graph(lym_ctx *ctx, int on);
  * NULL: Philox-4x32-10 uniforms, counter {mu*V+s, sweep, hit, k/4}, key seed.
  * predrawn != NULL (test mode): uniform k of hit h of link l in sweep w is
  * predrawn[((w*4V + l)*nhit + h)*n + k], n = 13 (LYM_CAND_SU2) or 9 (LYM_CAND_EXPH).
- * Extents must be even.  *accepted = number of accepted hits.  The sweeps
- * run on a copy of the links in even-odd site order held in the context's
- * second link buffer; the updated links are back in the usual order when the
- * call returns (it synchronises the context's stream). */
+ * Extents must be even.  *accepted = number of accepted hits; the call
+ * synchronises the context's stream.  The sweeps run on the links reordered
+ * into even-odd site order (a copy into the context's second link buffer),
+ * and the context keeps them in that order until another call reads them,
+ * which reorders them back first: consecutive calls, one sweep each as the
+ * reference's MCUpdate, pay no reorder. */
 int lym_metropolis(lym_ctx *ctx, double beta, double eps, int nhit, int nsweeps, int first_sweep,
                    int candidate, uint64_t seed, const double *predrawn, int64_t *accepted);
 

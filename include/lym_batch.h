@@ -36,8 +36,8 @@ int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds);
 /* Checkerboard Metropolis on every lattice at once (GaugeFieldFactory::MCUpdate,
  * src/Apps/gaugefieldfactory.cpp:192-220), Philox keyed by seeds[c];
  * accepted[c] = accepted hits of lattice c (may be NULL).  Each lattice gets
- * the same updates as lym_metropolis on its own context; like it, the sweeps
- * run on an even-odd copy in the second link buffer. */
+ * the same updates as lym_metropolis on its own context, and like it the
+ * batch keeps the links in even-odd order until another call reads them. */
 int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int nsweeps, int first_sweep,
                          int candidate, const uint64_t *seeds, int64_t *accepted);
 /* SuperObs::compute for every lattice (src/Observables/superobs.cpp:31-128):

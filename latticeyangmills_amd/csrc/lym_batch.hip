@@ -210,6 +210,7 @@ struct lym_batch {
     hipStream_t stream = nullptr;
     void *W[2] = {nullptr, nullptr};
     int cur = 0;
+    bool eo = false;  // W[cur] in even-odd order (left so by lym_batch_metropolis)
     void *acc = nullptr;
     ksum *partials = nullptr;
     double *obs = nullptr;
@@ -502,6 +503,41 @@ int lym_batch_info(const lym_batch *b, int *nconf, int64_t *total_sites, int *gr
     return LYM_OK;
 }
 
+// Copies W[cur] into W[1 - cur] reordered to (to_eo) or from the even-odd
+// order of the Metropolis sweeps, every lattice in one launch.
+static void batch_reorder_eo(lym_batch *b, bool to_eo)
+{
+    const dim3 grid(nblocks(b->total_sites, 256), 36);
+    if (b->prec == LYM_PREC_FP64) {
+        auto *in = static_cast<const double2 *>(b->W[b->cur]);
+        auto *out = static_cast<double2 *>(b->W[1 - b->cur]);
+        if (to_eo)
+            lym::k_batch_reorder_eo<double, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+        else
+            lym::k_batch_reorder_eo<double, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+    } else {
+        auto *in = static_cast<const float2 *>(b->W[b->cur]);
+        auto *out = static_cast<float2 *>(b->W[1 - b->cur]);
+        if (to_eo)
+            lym::k_batch_reorder_eo<float, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+        else
+            lym::k_batch_reorder_eo<float, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
+    }
+}
+
+// As the single-lattice context: the links stay in even-odd order after
+// lym_batch_metropolis until an entry point reads them site by site.
+static int to_order(lym_batch *b, bool eo)
+{
+    if (b->eo == eo) return LYM_OK;
+    batch_reorder_eo(b, eo);
+    BLAUNCHED(b, "k_batch_reorder_eo");
+    b->cur ^= 1;
+    b->eo = eo;
+    return LYM_OK;
+}
+static int to_lex(lym_batch *b) { return to_order(b, false); }
+
 static int check_conf(lym_batch *b, int c)
 {
     if (c < 0 || c >= b->nconf) return bfail(b, LYM_ERR_ARG, "configuration index out of range");
@@ -513,6 +549,7 @@ int lym_batch_upload_links(lym_batch *b, int c, const double *const by_mu[4])
     if (!b || !by_mu) return LYM_ERR_ARG;
     if (check_conf(b, c)) return LYM_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
+    if (int rc = to_lex(b)) return rc;  // the other lattices keep their links
     const ConfDesc &d = b->desc[c];
     const long V = d.g.V;
     for (int mu = 0; mu < 4; mu++) {
@@ -535,6 +572,7 @@ int lym_batch_download_links(lym_batch *b, int c, double *const by_mu[4])
     if (!b || !by_mu) return LYM_ERR_ARG;
     if (check_conf(b, c)) return LYM_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
+    if (int rc = to_lex(b)) return rc;
     const ConfDesc &d = b->desc[c];
     const long V = d.g.V;
     for (int mu = 0; mu < 4; mu++) {
@@ -566,30 +604,9 @@ int lym_batch_set_hot(lym_batch *b, const uint64_t *seeds)
                 static_cast<float2 *>(b->W[b->cur]) + b->desc[c].link_off, V, seeds[c]);
         BLAUNCHED(b, "k_hot");
     }
+    b->eo = false;
     BHIP(b, hipStreamSynchronize(b->stream));
     return LYM_OK;
-}
-
-// Copies W[cur] into W[1 - cur] reordered to (to_eo) or from the even-odd
-// order of the Metropolis sweeps, every lattice in one launch.
-static void batch_reorder_eo(lym_batch *b, bool to_eo)
-{
-    const dim3 grid(nblocks(b->total_sites, 256), 36);
-    if (b->prec == LYM_PREC_FP64) {
-        auto *in = static_cast<const double2 *>(b->W[b->cur]);
-        auto *out = static_cast<double2 *>(b->W[1 - b->cur]);
-        if (to_eo)
-            lym::k_batch_reorder_eo<double, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
-        else
-            lym::k_batch_reorder_eo<double, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
-    } else {
-        auto *in = static_cast<const float2 *>(b->W[b->cur]);
-        auto *out = static_cast<float2 *>(b->W[1 - b->cur]);
-        if (to_eo)
-            lym::k_batch_reorder_eo<float, true><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
-        else
-            lym::k_batch_reorder_eo<float, false><<<grid, 256, 0, b->stream>>>(in, out, b->d_desc, b->nconf, b->total_sites);
-    }
 }
 
 extern "C++" template <typename S>
@@ -620,12 +637,9 @@ int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int ns
     BHIP(b, hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(ConfDesc) * b->nconf, hipMemcpyHostToDevice, b->stream));
     lym::k_zero_u64<<<nblocks(b->nconf, 64), 64, 0, b->stream>>>(b->accepted, b->nconf);
     const int nu = candidate == LYM_CAND_SU2 ? 13 : 9;
-    // the sweeps run on an even-odd copy in the other buffer, reordered back at the end
-    if (nsweeps > 0) {
-        batch_reorder_eo(b, true);
-        BLAUNCHED(b, "k_batch_reorder_eo");
-        b->cur ^= 1;
-    }
+    // the sweeps run on the links in even-odd order, which they are left in
+    if (nsweeps > 0)
+        if (int rc = to_order(b, true)) return rc;
     const unsigned nb = nblocks(b->nitems, 256);
     for (int sw = 0; sw < nsweeps; sw++) {
         lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, 0, nullptr};
@@ -638,11 +652,6 @@ int lym_batch_metropolis(lym_batch *b, double beta, double eps, int nhit, int ns
                     launch_batch_metropolis<float>(b, nb, mu, p);
                 BLAUNCHED(b, "k_batch_metropolis");
             }
-    }
-    if (nsweeps > 0) {
-        batch_reorder_eo(b, false);
-        BLAUNCHED(b, "k_batch_reorder_eo");
-        b->cur ^= 1;
     }
     std::vector<unsigned long long> acc(b->nconf);
     BHIP(b, hipMemcpyAsync(acc.data(), b->accepted, sizeof(unsigned long long) * b->nconf, hipMemcpyDeviceToHost,
@@ -657,7 +666,8 @@ int lym_batch_observables(lym_batch *b, double *out)
 {
     if (!b || !out) return LYM_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
-    int rc = observables(b, 0);
+    int rc = to_lex(b);
+    if (!rc) rc = observables(b, 0);
     if (rc) return rc;
     std::vector<double> raw(4 * size_t(b->nconf));
     BHIP(b, hipMemcpyAsync(raw.data(), b->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, b->stream));
@@ -687,8 +697,8 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
     if (rows < 1 || rows > max_rows) return bfail(b, LYM_ERR_ARG, "series too small for int(tau/eps) rows");
     BHIP(b, hipSetDevice(b->device));
     const int nmeas = 1 + (rows - 1) / measure_every;
-    int rc = ensure_slots(b, nmeas);
-    if (rc) return rc;
+    int rc = to_lex(b);
+    if (rc || (rc = ensure_slots(b, nmeas))) return rc;
     const int nsteps = rows - 1;
     if (use_graph) {
         // the graph is recorded for fixed buffers: W[cur] -> ... ; the final
@@ -755,6 +765,7 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
 {
     if (!b) return LYM_ERR_ARG;
     BHIP(b, hipSetDevice(b->device));
+    if (int rc = to_lex(b)) return rc;
     const std::string key = shape_key(b);
     std::map<std::string, std::string> cache;
     if (cache_path) {

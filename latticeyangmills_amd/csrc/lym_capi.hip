@@ -25,6 +25,7 @@ struct lym_ctx {
     hipStream_t stream = nullptr;
     void *W[2] = {nullptr, nullptr};  // double-buffered link field (SoA)
     int cur = 0;                      // W[cur] holds the current links
+    bool eo = false;                  // ... in even-odd order (left so by lym_metropolis)
     void *acc = nullptr;              // RK accumulator, 8 reals per link
     ksum *partials = nullptr;         // per-workgroup observables partials
     int nblk_obs = 0;
@@ -93,6 +94,44 @@ unsigned stencil_blocks(const lym_ctx *c)
 {
     return c->sched.order.empty() ? blocks_for(c->g.V, lym::kSitesPerBlock) : unsigned(lym::kXcds * c->sched.per_xcd);
 }
+
+// Copies W[cur] into W[1 - cur] reordered to (to_eo) or from the even-odd
+// order of the Metropolis sweeps (lym::EOLinks).
+void reorder_eo(lym_ctx *c, bool to_eo)
+{
+    const dim3 grid(blocks_for(c->g.V, 256), 36);
+    if (c->prec == LYM_PREC_FP64) {
+        auto *in = static_cast<const double2 *>(c->W[c->cur]);
+        auto *out = static_cast<double2 *>(c->W[1 - c->cur]);
+        if (to_eo)
+            lym::k_reorder_eo<double, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+        else
+            lym::k_reorder_eo<double, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+    } else {
+        auto *in = static_cast<const float2 *>(c->W[c->cur]);
+        auto *out = static_cast<float2 *>(c->W[1 - c->cur]);
+        if (to_eo)
+            lym::k_reorder_eo<float, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+        else
+            lym::k_reorder_eo<float, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
+    }
+}
+
+// The links in W[cur] are in lexicographic order, except after
+// lym_metropolis, which leaves them in even-odd order so that consecutive
+// calls (one sweep each, as the reference's MCUpdate) skip both reorders.
+// Every entry point that reads them site by site calls to_lex first; those
+// that overwrite them all clear the flag.
+int to_order(lym_ctx *c, bool eo)
+{
+    if (c->eo == eo) return LYM_OK;
+    reorder_eo(c, eo);
+    LYM_LAUNCHED(c, "k_reorder_eo");
+    c->cur ^= 1;
+    c->eo = eo;
+    return LYM_OK;
+}
+int to_lex(lym_ctx *c) { return to_order(c, false); }
 
 int apply_schedule(lym_ctx *c, int kind, int target)
 {
@@ -307,6 +346,7 @@ int lym_upload_links(lym_ctx *c, const double *const by_mu[4])
                 c->staging, static_cast<float2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
         LYM_LAUNCHED(c, "k_aos_to_soa");
     }
+    c->eo = false;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -317,6 +357,7 @@ int lym_download_links(lym_ctx *c, double *const by_mu[4])
     for (int mu = 0; mu < 4; mu++)
         if (!by_mu[mu]) return fail(c, LYM_ERR_ARG, "null link array");
     LYM_HIP(c, hipSetDevice(c->device));
+    if (int rc = to_lex(c)) return rc;
     const long V = c->g.V;
     for (int mu = 0; mu < 4; mu++) {
         if (c->prec == LYM_PREC_FP64)
@@ -342,6 +383,7 @@ int lym_set_cold(lym_ctx *c)
     else
         lym::k_cold<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V);
     LYM_LAUNCHED(c, "k_cold");
+    c->eo = false;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -356,6 +398,7 @@ int lym_set_hot(lym_ctx *c, uint64_t seed)
     else
         lym::k_hot<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V, seed);
     LYM_LAUNCHED(c, "k_hot");
+    c->eo = false;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -364,6 +407,7 @@ static int export_force(lym_ctx *c, int mu, int what, double *out)
 {
     if (!c || !out || mu < 0 || mu > 3) return c ? fail(c, LYM_ERR_ARG, "bad mu or null output") : LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
+    if (int rc = to_lex(c)) return rc;
     const long V = c->g.V;
     if (c->prec == LYM_PREC_FP64)
         lym::k_export_force<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
@@ -384,7 +428,8 @@ int lym_observables(lym_ctx *c, double out[4])
 {
     if (!c || !out) return LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
-    int rc = observables(c, 0);
+    int rc = to_lex(c);
+    if (rc || (rc = observables(c, 0))) return rc;
     if (rc) return rc;
     double raw[4];
     LYM_HIP(c, hipMemcpyAsync(raw, c->obs, sizeof raw, hipMemcpyDeviceToHost, c->stream));
@@ -397,6 +442,7 @@ int lym_flow_steps(lym_ctx *c, double eps, int nsteps)
 {
     if (!c || nsteps < 0 || !(eps > 0)) return c ? fail(c, LYM_ERR_ARG, "bad eps or nsteps") : LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
+    if (int rc = to_lex(c)) return rc;
     for (int i = 0; i < nsteps; i++) {
         int rc = step(c, eps);
         if (rc) return rc;
@@ -413,8 +459,8 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
     if (rows < 1 || rows > max_rows) return fail(c, LYM_ERR_ARG, "series too small for int(tau/eps) rows");
     LYM_HIP(c, hipSetDevice(c->device));
     const int nmeas = 1 + (rows - 1) / measure_every;
-    int rc = ensure_slots(c, nmeas);
-    if (rc) return rc;
+    int rc = to_lex(c);
+    if (rc || (rc = ensure_slots(c, nmeas))) return rc;
     auto enqueue = [&]() -> int {
         int slot = 0, r = observables(c, slot++);
         for (int t = 1; t < rows && !r; t++) {
@@ -484,28 +530,6 @@ int lym_set_graph(lym_ctx *c, int on)
     return LYM_OK;
 }
 
-// Copies W[cur] into W[1 - cur] reordered to (TO_EO) or from the even-odd
-// order of the Metropolis sweeps.
-static void reorder_eo(lym_ctx *c, bool to_eo)
-{
-    const dim3 grid(blocks_for(c->g.V, 256), 36);
-    if (c->prec == LYM_PREC_FP64) {
-        auto *in = static_cast<const double2 *>(c->W[c->cur]);
-        auto *out = static_cast<double2 *>(c->W[1 - c->cur]);
-        if (to_eo)
-            lym::k_reorder_eo<double, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
-        else
-            lym::k_reorder_eo<double, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
-    } else {
-        auto *in = static_cast<const float2 *>(c->W[c->cur]);
-        auto *out = static_cast<float2 *>(c->W[1 - c->cur]);
-        if (to_eo)
-            lym::k_reorder_eo<float, true><<<grid, 256, 0, c->stream>>>(in, out, c->g);
-        else
-            lym::k_reorder_eo<float, false><<<grid, 256, 0, c->stream>>>(in, out, c->g);
-    }
-}
-
 // One Metropolis sub-sweep launch; the candidate kind and the test mode pick
 // the kernel instantiation, so each carries only its own registers.
 extern "C++" template <typename S>
@@ -557,13 +581,10 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     }
     lym::k_zero_u64<<<1, 64, 0, c->stream>>>(c->accepted, 1);
     LYM_LAUNCHED(c, "k_zero_u64");
-    // The sweeps run on an even-odd copy of the links in the other buffer,
-    // which is reordered back at the end.
-    if (nsweeps > 0) {
-        reorder_eo(c, true);
-        LYM_LAUNCHED(c, "k_reorder_eo");
-        c->cur ^= 1;
-    }
+    // The sweeps run on the links in even-odd order, which they are left in
+    // (to_lex reorders them when another entry point reads them).
+    if (nsweeps > 0)
+        if (int rc = to_order(c, true)) return rc;
     const unsigned nb = blocks_for(V / 2, 256);
     for (int sw = 0; sw < nsweeps; sw++) {
         lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, seed,
@@ -578,11 +599,6 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
                 LYM_LAUNCHED(c, "k_metropolis");
             }
     }
-    if (nsweeps > 0) {
-        reorder_eo(c, false);
-        LYM_LAUNCHED(c, "k_reorder_eo");
-        c->cur ^= 1;
-    }
     unsigned long long acc = 0;
     LYM_HIP(c, hipMemcpyAsync(&acc, c->accepted, sizeof acc, hipMemcpyDeviceToHost, c->stream));
     LYM_HIP(c, hipStreamSynchronize(c->stream));
@@ -593,6 +609,7 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
 int lym_check_links(lym_ctx *c, double *max_dev, int64_t *nonfinite)
 {
     if (!c) return LYM_ERR_ARG;
+    // a maximum and a count over all links: either site order will do
     unsigned long long *d = c->check;
     lym::k_zero_u64<<<1, 64, 0, c->stream>>>(d, 2);
     LYM_LAUNCHED(c, "k_zero_u64");
@@ -625,8 +642,8 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
     LYM_HIP(c, hipSetDevice(c->device));
     const int nmeas = measure_every ? nsteps / measure_every : 0;
     const size_t nev = 2 + 4 * size_t(nsteps) + 2 * size_t(nmeas);
-    int rc = ensure_events(c, nev);
-    if (rc) return rc;
+    int rc = to_lex(c);
+    if (rc || (rc = ensure_events(c, nev))) return rc;
     hipEvent_t *ev = c->events.data();
     LYM_HIP(c, hipEventRecord(ev[0], c->stream));
     size_t k = 2;
@@ -680,6 +697,7 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
 {
     if (!c) return LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
+    if (int rc = to_lex(c)) return rc;
     std::ostringstream key;
     key << "gfx950|single|prec" << c->prec << "|" << c->dims[0] << "x" << c->dims[1] << "x" << c->dims[2] << "x"
         << c->dims[3];

@@ -153,3 +153,29 @@ def test_slab_staging_batch(precision, grid, dims):
             s, links = singles[c]
             assert np.abs(b.download(c) - links).max() < (1e-13 if fp64 else 2e-6)
             assert np.allclose(series[c][:, [0, 1, 3]], s[:, [0, 1, 3]], rtol=1e-13 if fp64 else 1e-7, atol=0)
+
+
+def test_batch_even_odd_order_is_invisible(inputs):
+    """The batch keeps its links in even-odd order after lym_batch_metropolis:
+    one-sweep calls equal one two-sweep call, an upload of one lattice leaves
+    the others' updated links in place, and observables and downloads read
+    the lexicographic links."""
+    seeds = list(range(31, 31 + len(DIMS)))
+    with BatchContext(DIMS, "fp64") as a, BatchContext(DIMS, "fp64") as b:
+        for c, L in enumerate(inputs):
+            a.upload(c, L)
+            b.upload(c, L)
+        a.metropolis(6.0, 0.24, 3, 1, seeds, first_sweep=0)
+        a.metropolis(6.0, 0.24, 3, 1, seeds, first_sweep=1)
+        b.metropolis(6.0, 0.24, 3, 2, seeds, first_sweep=0)
+        want = [b.download(c) for c in range(len(DIMS))]
+        a.upload(0, inputs[0])  # after sweeps: lattice 0 replaced, the rest kept
+        assert np.array_equal(a.download(0), inputs[0])
+        for c in range(1, len(DIMS)):
+            np.testing.assert_array_equal(a.download(c), want[c])
+        b.upload(0, inputs[0])
+        a.metropolis(6.0, 0.24, 3, 1, seeds, first_sweep=2)
+        b.metropolis(6.0, 0.24, 3, 1, seeds, first_sweep=2)
+        for c in range(len(DIMS)):
+            b.upload(c, b.download(c))
+        np.testing.assert_array_equal(a.observables(), b.observables())

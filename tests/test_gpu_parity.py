@@ -324,3 +324,47 @@ def test_flow_graph_replays_match_eager(precision):
             a.synchronize()
         b.set_graph(False)
         np.testing.assert_array_equal(b.flow(eps, tau, every)[:, 1:3], a.flow(eps, tau, every)[:, 1:3])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_even_odd_order_is_invisible(precision):
+    """lym_metropolis leaves the links in even-odd order inside the context
+    until another call reads them.  One-sweep calls interleaved with every
+    reading entry point equal a context that runs the same sweeps in one call
+    and passes the links through host memory before each reading call."""
+    dims = (4, 4, 4, 6)
+    links = orc.hot_start(dims, 53)
+    with LatticeContext(dims, precision) as a, LatticeContext(dims, precision) as b:
+        a.upload(links)
+        b.upload(links)
+        a.metropolis(6.0, 0.24, 3, 1, 0, "exph", 9)
+        a.metropolis(6.0, 0.24, 3, 1, 1, "exph", 9)  # already in even-odd order
+        b.metropolis(6.0, 0.24, 3, 2, 0, "exph", 9)
+        b.upload(b.download())
+        assert np.array_equal(a.observables(), b.observables())
+        a.metropolis(6.0, 0.24, 3, 1, 2, "exph", 9)
+        b.metropolis(6.0, 0.24, 3, 1, 2, "exph", 9)
+        b.upload(b.download())
+        np.testing.assert_array_equal(a.staples(1), b.staples(1))
+        a.metropolis(6.0, 0.24, 3, 1, 3, "exph", 9)
+        b.metropolis(6.0, 0.24, 3, 1, 3, "exph", 9)
+        b.upload(b.download())
+        np.testing.assert_array_equal(a.derivative(2), b.derivative(2))
+        a.metropolis(6.0, 0.24, 3, 1, 4, "exph", 9)
+        b.metropolis(6.0, 0.24, 3, 1, 4, "exph", 9)
+        assert a.check_links() == b.check_links()  # order-free: a max and a count
+        b.upload(b.download())
+        a.flow_steps(0.02, 3)
+        b.flow_steps(0.02, 3)
+        a.metropolis(6.0, 0.24, 3, 1, 5, "exph", 9)
+        b.metropolis(6.0, 0.24, 3, 1, 5, "exph", 9)
+        b.upload(b.download())
+        sa, sb = a.flow(0.02, 0.1, 1), b.flow(0.02, 0.1, 1)
+        np.testing.assert_array_equal(sa, sb)
+        a.metropolis(6.0, 0.24, 3, 1, 6, "exph", 9)
+        b.metropolis(6.0, 0.24, 3, 1, 6, "exph", 9)
+        np.testing.assert_array_equal(a.download(), b.download())
+        a.metropolis(6.0, 0.24, 3, 1, 7, "exph", 9)
+        a.set_hot(5)  # overwrites every link: no reorder owed
+        b.set_hot(5)
+        np.testing.assert_array_equal(a.download(), b.download())

@@ -17,6 +17,13 @@
 using lym::Geom;
 using lym::ksum;
 
+// Metropolis blocks in column order (lym::metro_column_schedule): each XCD
+// marches (z, y) columns of about this many sites of one parity along t, so
+// the neighbouring time slices of a column stay in its 4 MB L2.  A time
+// slice of 32^3 x 64 in even-odd order is 9.4 MB per parity (FP64), which
+// the XCD-contiguous order refetches for every slice it updates.
+constexpr int kMetroColumnSites = 4096;
+
 struct lym_ctx {
     int device = 0;
     int prec = LYM_PREC_FP64;
@@ -39,6 +46,9 @@ struct lym_ctx {
     std::vector<hipEvent_t> events;
     lym::HostSchedule sched;  // work schedule of the stencil kernels
     int *d_order = nullptr;
+    int metro_target = kMetroColumnSites;  // Metropolis block order: 0 = XCD-contiguous, else columns
+    int *d_morder = nullptr;               // its table (built on first use)
+    int morder_per_xcd = 0;
     // lym_flow as a captured graph (lym_set_graph), one per start buffer
     // W[cur]; a graph holds the buffer pointers of its capture, so every
     // reallocation of d_order, partials or obs drops both
@@ -309,7 +319,8 @@ int lym_destroy(lym_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
-                    (void *)c->accepted, (void *)c->check, (void *)c->predrawn, (void *)c->d_order})
+                    (void *)c->accepted, (void *)c->check, (void *)c->predrawn, (void *)c->d_order,
+                    (void *)c->d_morder})
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -535,23 +546,24 @@ int lym_set_graph(lym_ctx *c, int on)
 extern "C++" template <typename S>
 static void launch_metropolis(lym_ctx *c, unsigned nb, int mu, const lym::MetroParams &p)
 {
-    const unsigned per_xcd = (nb + 7) / 8;
+    const unsigned per_xcd = c->d_morder ? unsigned(c->morder_per_xcd) : (nb + 7) / 8;
+    const int *ord = c->d_morder;
     auto *f = static_cast<lym::F2<S> *>(c->W[c->cur]);
     if (p.candidate == LYM_CAND_SU2) {
         if (p.predrawn)
-            lym::k_metropolis<S, 0, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 0, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 0, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 0, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
     } else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max) {
         if (p.predrawn)
-            lym::k_metropolis<S, 1, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 1, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 1, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 1, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
     } else {  // every candidate has c1 < 8 eps^2: series-only exponential
         if (p.predrawn)
-            lym::k_metropolis<S, 2, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 2, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
         else
-            lym::k_metropolis<S, 2, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, per_xcd, c->accepted);
+            lym::k_metropolis<S, 2, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
     }
 }
 
@@ -585,6 +597,13 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     // (to_lex reorders them when another entry point reads them).
     if (nsweeps > 0)
         if (int rc = to_order(c, true)) return rc;
+    if (c->metro_target > 0 && !c->d_morder) {
+        const lym::HostSchedule h =
+            lym::metro_column_schedule({c->dims[0], c->dims[1], c->dims[2], c->dims[3]}, c->metro_target);
+        LYM_HIP(c, hipMalloc(&c->d_morder, sizeof(int) * h.order.size()));
+        LYM_HIP(c, hipMemcpy(c->d_morder, h.order.data(), sizeof(int) * h.order.size(), hipMemcpyHostToDevice));
+        c->morder_per_xcd = h.per_xcd;
+    }
     const unsigned nb = blocks_for(V / 2, 256);
     for (int sw = 0; sw < nsweeps; sw++) {
         lym::MetroParams p{beta, eps, nhit, candidate, nu, uint32_t(first_sweep + sw), 0, seed,

@@ -1006,10 +1006,13 @@ __global__ __launch_bounds__(256) void k_reorder_eo(const F2<S> *__restrict__ in
 // lattice return at once.
 template <typename S, int CAND, bool PREDRAWN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(
-    F2<S> *__restrict__ f, Geom g, int mu, MetroParams p, unsigned per_xcd, unsigned long long *__restrict__ accepted)
+    F2<S> *__restrict__ f, Geom g, int mu, MetroParams p, const int *__restrict__ order, unsigned per_xcd,
+    unsigned long long *__restrict__ accepted)
 {
-    const unsigned blk = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
-    const long i = (long)blk * 256 + threadIdx.x;
+    const unsigned slot = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const long blk = order ? long(order[slot]) : long(slot);  // -1: padding
+    if (blk < 0) return;
+    const long i = blk * 256 + threadIdx.x;
     int acc = 0;
     if (i < g.V / 2) acc = metro_link_mu<S, CAND, PREDRAWN>(f, g, mu, parity_site(g, i, p.parity), i, p);
 #pragma unroll

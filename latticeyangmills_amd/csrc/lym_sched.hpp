@@ -95,6 +95,39 @@ inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, 
     return h;
 }
 
+// Metropolis blocks (256 consecutive sites of one parity in even-odd order,
+// block q = sites [256 q, 256 q + 256) of the parity) in COLUMN order: a
+// column is a block of zc x yc rows of the (z, y) plane holding about
+// `target` sites of one parity, marched along t; then split into 8
+// contiguous parts, one per XCD, as build_schedule.
+inline HostSchedule metro_column_schedule(const std::array<int, 4> &L, int target)
+{
+    const long half_x = L[0] / 2, nsites = long(L[0]) * L[1] * L[2] * L[3] / 2;
+    const long nblk = (nsites + 255) / 256;
+    const int rows = std::max(1, int(target / half_x));
+    const int yc = std::min(L[1], std::max(1, int(std::lround(std::sqrt(double(rows))))));
+    const int zc = std::min(L[2], std::max(1, rows / yc));
+    const int ny = (L[1] + yc - 1) / yc;
+    std::vector<std::pair<uint64_t, int>> key(static_cast<size_t>(nblk));
+    for (long q = 0; q < nblk; q++) {
+        const long r = q * 256 / half_x;
+        const int y = int(r % L[1]), z = int(r / L[1] % L[2]), t = int(r / (long(L[1]) * L[2]));
+        const uint64_t col = uint64_t(z / zc) * ny + uint64_t(y / yc);
+        key[size_t(q)] = {((col * L[3] + t) * L[2] + z) * uint64_t(nblk) + uint64_t(q), int(q)};
+    }
+    std::sort(key.begin(), key.end());
+    HostSchedule h;
+    h.kind = SCHED_COLUMN;
+    h.target = target;
+    h.per_xcd = int((nblk + kXcds - 1) / kXcds);
+    h.order.assign(size_t(kXcds) * h.per_xcd, -1);
+    for (int x = 0; x < kXcds; x++) {
+        const long b = nblk * x / kXcds, e = nblk * (x + 1) / kXcds;
+        for (long i = b; i < e; i++) h.order[size_t(x) * h.per_xcd + size_t(i - b)] = key[size_t(i)].second;
+    }
+    return h;
+}
+
 // the candidates the autotuners time: every order, with and without LDS staging
 inline std::vector<std::pair<int, int>> schedule_candidates()
 {

@@ -174,6 +174,22 @@ def test_metropolis_philox_per_link(candidate, eps):
     assert np.max(np.abs(got - want)) < 1e-11
 
 
+def test_metropolis_philox_per_link_column_order():
+    """A lattice whose (z, y) planes hold more than one Metropolis column
+    (32 x 32 x 32 x 4: four columns of 4096 sites per parity, each marched
+    along t by one XCD): the block order leaves every link update as the
+    oracle's."""
+    dims = (32, 32, 32, 4)
+    links = orc.hot_start(dims, 6)
+    want, acc_want = orc.metropolis(links, dims, 6.0, 0.24, 2, 1, 0, 1, 77)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        acc = ctx.metropolis(6.0, 0.24, 2, 1, 0, "exph", 77)
+        got = ctx.download()
+    assert acc == acc_want
+    assert np.max(np.abs(got - want)) < 1e-11
+
+
 def test_mixed_precision_against_fp64(therm8):
     """FP32 links/staples/clover, FP64 exp and reductions: error stated vs FP64."""
     dims, links = therm8

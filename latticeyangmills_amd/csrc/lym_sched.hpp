@@ -10,9 +10,10 @@
 //   LINEAR  no table: workgroup b takes run b (each XCD sees every 8th run);
 //   CONTIG  the run list split into 8 contiguous parts, one per XCD;
 //   COLUMN  runs ordered by (column, t, z, y, x) where a column is a block of
-//           Zc x Yc rows of the (z, y) plane of about `target` sites, so an XCD
-//           marches a narrow column along t and the previous and next time
-//           slices of the column stay in its L2; then split as CONTIG.
+//           Zc x Yc rows of the (z, y) plane of about `target` sites (Yc | Ly,
+//           Zc | Lz), so an XCD marches a narrow column along t and the
+//           previous and next time slices of the column stay in its L2; then
+//           split as CONTIG.
 // Any schedule visits every run once, so results do not depend on it.
 #pragma once
 #include <algorithm>
@@ -35,15 +36,34 @@ struct HostSchedule {
     std::vector<int> order;  // kXcds * per_xcd entries (empty for LINEAR)
 };
 
+// Column cross-section (yc, zc) in (y, z) rows: divisors of Ly and Lz, so
+// that every column is whole, with yc * zc closest to `rows` (in ratio) and,
+// between equals, nearest to square (the least edge per site).
+inline std::pair<int, int> column_shape(int Ly, int Lz, int rows)
+{
+    int by = 1, bz = 1;
+    double best = 1e300;
+    for (int yc = 1; yc <= Ly; yc++)
+        for (int zc = 1; zc <= Lz; zc++) {
+            if (Ly % yc || Lz % zc) continue;
+            const double cost = std::fabs(std::log(double(yc) * zc / rows)) + 0.05 * std::fabs(std::log(double(yc) / zc));
+            if (cost < best) {
+                best = cost;
+                by = yc;
+                bz = zc;
+            }
+        }
+    return {by, bz};
+}
+
 // runs of one lattice in COLUMN order (run indices relative to the lattice)
 inline std::vector<int> column_order(const std::array<int, 4> &L, int target)
 {
     const long V = long(L[0]) * L[1] * L[2] * L[3];
     const long nruns = (V + kRunSites - 1) / kRunSites;
     const int rows = std::max(1, target / L[0]);  // (y, z) rows per column
-    const int yc = std::min(L[1], std::max(1, int(std::lround(std::sqrt(double(rows))))));
-    const int zc = std::min(L[2], std::max(1, rows / yc));
-    const int ny = (L[1] + yc - 1) / yc;
+    const auto [yc, zc] = column_shape(L[1], L[2], rows);
+    const int ny = L[1] / yc;
     const size_t n = static_cast<size_t>(nruns);
     std::vector<std::pair<uint64_t, int>> key(n);
     for (long r = 0; r < nruns; r++) {
@@ -105,9 +125,8 @@ inline HostSchedule metro_column_schedule(const std::array<int, 4> &L, int targe
     const long half_x = L[0] / 2, nsites = long(L[0]) * L[1] * L[2] * L[3] / 2;
     const long nblk = (nsites + 255) / 256;
     const int rows = std::max(1, int(target / half_x));
-    const int yc = std::min(L[1], std::max(1, int(std::lround(std::sqrt(double(rows))))));
-    const int zc = std::min(L[2], std::max(1, rows / yc));
-    const int ny = (L[1] + yc - 1) / yc;
+    const auto [yc, zc] = column_shape(L[1], L[2], rows);
+    const int ny = L[1] / yc;
     std::vector<std::pair<uint64_t, int>> key(static_cast<size_t>(nblk));
     for (long q = 0; q < nblk; q++) {
         const long r = q * 256 / half_x;

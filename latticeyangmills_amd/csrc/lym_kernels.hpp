@@ -999,10 +999,13 @@ __global__ __launch_bounds__(256) void k_reorder_eo(const F2<S> *__restrict__ in
 }
 
 // One Metropolis sub-sweep (direction mu, one parity) over a field in
-// even-odd order.  Workgroups are dispatched round-robin over the 8 XCDs;
-// workgroup b works on block (b % 8) * per_xcd + b / 8, so each XCD sweeps
-// one contiguous range of the lattice and the neighbour links along y and z
-// stay in its L2.  The grid is 8 * per_xcd workgroups; blocks past the
+// even-odd order; block q = sites [256 q, 256 q + 256) of the parity.
+// Workgroups are dispatched round-robin over the 8 XCDs, so workgroup b runs
+// on XCD b % 8 and takes slot (b % 8) * per_xcd + b / 8 of the block order:
+// order[slot] from the host's column table (metro_column_schedule: each XCD
+// marches (z, y) columns along t, so the neighbouring time slices stay in its
+// L2), or the slot itself when order is null (each XCD one contiguous range).
+// The grid is 8 * per_xcd workgroups; padding slots (-1) and blocks past the
 // lattice return at once.
 template <typename S, int CAND, bool PREDRAWN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAND == 1 ? 1 : 2))) void k_metropolis(

@@ -329,7 +329,7 @@ int or_flow_step(double *links, const int dims[4], double eps)
  * the flow runs rows-1 steps (wilsonflow.cpp:124). */
 int or_flow_rows(double tau, double eps) { return (int)(tau / eps); }
 
-/* Flow driver (src/Apps/wilsonflow.cpp:115-136).  series has rows*4 entries,
+/* Flow loop (src/Apps/wilsonflow.cpp:115-136).  series has rows*4 entries,
  * columns [t, plaq, topc, energy]; t accumulates by += eps.  With
  * measure_every = k only rows with t % k == 0 are measured (others NaN);
  * the reference measures every row (k = 1). */

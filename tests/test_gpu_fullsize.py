@@ -1,6 +1,6 @@
-"""The BASELINE workloads at their real sizes.
+"""The benchmark workloads at their real sizes: 8^4, 24^4, 32^3x64, 48^3x96.
 
-cfg1 (8^4) runs end to end against the oracle: 200 Philox Metropolis sweeps
+8^4 runs end to end against the oracle: 200 Philox Metropolis sweeps
 from a cold start compared link for link, then the flow to t = 1.0 row by row.
 Where the oracle is too slow for a whole run (24^4 flows for 399 steps), the
 first rows are compared exactly and the rest through size-independent
@@ -25,7 +25,7 @@ def unitarity(links):
     return float(np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max())
 
 
-def test_cfg1_pipeline_against_oracle():
+def test_8x8x8x8_pipeline_against_oracle():
     """8^4, beta 6.0, cold, 200 sweeps (10 hits, eps 0.24, seed 1234), flow eps 0.01 to t 1.0 every 10."""
     dims = (8, 8, 8, 8)
     cold = orc.cold_start(dims)
@@ -49,7 +49,7 @@ def test_cfg1_pipeline_against_oracle():
         assert abs(series_g[r, 2] - series_o[r, 2]) < 1e-12 * scale, r
 
 
-def test_cfg2_first_rows_against_oracle():
+def test_24x24x24x24_first_rows_against_oracle():
     """24^4, beta 6.0, 500 sweeps from seed 7: the GPU config flowed by both paths."""
     dims = (24, 24, 24, 24)
     with LatticeContext(dims) as ctx:
@@ -66,7 +66,7 @@ def test_cfg2_first_rows_against_oracle():
         assert abs(series_g[r, 2] - series_o[r, 2]) < 1e-12 * scale, r
 
 
-def test_cfg2_whole_flow_properties():
+def test_24x24x24x24_whole_flow_properties():
     dims = (24, 24, 24, 24)
     V = orc.volume(dims)
     with LatticeContext(dims) as ctx:
@@ -81,7 +81,7 @@ def test_cfg2_whole_flow_properties():
     assert unitarity(links) < 1e-12
 
 
-def test_cfg3_mixed_within_stated_error():
+def test_32x32x32x64_mixed_within_stated_error():
     """32^3x64, beta 6.2 (Haar start, 1000 sweeps, seed 62, as the bench): mixed
     flow vs FP64 flow of the same config to t = 5.98.  Bounds: <E> and the
     plaquette 1e-6 relative at every measured row, t0/a^2 1e-6 relative,
@@ -110,7 +110,7 @@ def test_cfg3_mixed_within_stated_error():
     assert u < 1e-6  # FP32 storage of reunitarised links
 
 
-def test_cfg5_production_volume():
+def test_48x48x48x96_production_volume():
     """48^3x96, beta 6.4: the largest single-GPU volume, mixed flow properties."""
     dims = (48, 48, 48, 96)
     with LatticeContext(dims, "fp64") as ctx:

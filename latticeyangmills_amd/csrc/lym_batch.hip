@@ -25,6 +25,7 @@
 #include "lym_batch.h"
 #include "lym_kernels.hpp"
 #include "lym_sched.hpp"
+#include "lym_xfer.hpp"
 
 namespace lym {
 
@@ -217,6 +218,7 @@ struct lym_batch {
     int obs_cap = 0;
     double *staging = nullptr;
     long staging_sites = 0;
+    lym::Bounce bounce;  // pinned buffers of lym_batch_upload_links / lym_batch_download_links
     unsigned long long *accepted = nullptr;
     int grid = 0;  // 0: one workgroup per schedule slot, else a persistent grid
     lym::HostSchedule sched;
@@ -482,6 +484,7 @@ int lym_batch_destroy(lym_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->bounce.release();
     for (void *p : {b->W[0], b->W[1], b->acc, (void *)b->partials, (void *)b->obs, (void *)b->staging,
                     (void *)b->accepted, (void *)b->d_desc, (void *)b->d_order})
         if (p) (void)hipFree(p);
@@ -554,7 +557,7 @@ int lym_batch_upload_links(lym_batch *b, int c, const double *const by_mu[4])
     const long V = d.g.V;
     for (int mu = 0; mu < 4; mu++) {
         if (!by_mu[mu]) return bfail(b, LYM_ERR_ARG, "null link array");
-        BHIP(b, hipMemcpyAsync(b->staging, by_mu[mu], sizeof(double) * 18 * V, hipMemcpyHostToDevice, b->stream));
+        BHIP(b, lym::h2d_bounced(b->bounce, b->stream, b->staging, by_mu[mu], sizeof(double) * 18 * V));
         if (b->prec == LYM_PREC_FP64)
             lym::k_aos_to_soa<double><<<nblocks(V, 256), 256, 0, b->stream>>>(
                 b->staging, static_cast<double2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, V);
@@ -584,8 +587,7 @@ int lym_batch_download_links(lym_batch *b, int c, double *const by_mu[4])
             lym::k_soa_to_aos<float><<<nblocks(V, 256), 256, 0, b->stream>>>(
                 static_cast<const float2 *>(b->W[b->cur]) + d.link_off + size_t(mu) * 9 * V, b->staging, V);
         BLAUNCHED(b, "k_soa_to_aos");
-        BHIP(b, hipMemcpyAsync(by_mu[mu], b->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, b->stream));
-        BHIP(b, hipStreamSynchronize(b->stream));
+        BHIP(b, lym::d2h_bounced(b->bounce, b->stream, by_mu[mu], b->staging, sizeof(double) * 18 * V));
     }
     return LYM_OK;
 }

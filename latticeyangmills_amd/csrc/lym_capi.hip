@@ -9,6 +9,7 @@
 #include "lym.h"
 #include "lym_kernels.hpp"
 #include "lym_sched.hpp"
+#include "lym_xfer.hpp"
 
 #include <fstream>
 #include <map>
@@ -39,6 +40,7 @@ struct lym_ctx {
     double *obs = nullptr;  // measurement slots, 4 doubles each
     int obs_cap = 0;
     double *staging = nullptr;  // V x 18 doubles (reference layout of one direction)
+    lym::Bounce bounce;         // pinned buffers of lym_upload_links / lym_download_links
     unsigned long long *accepted = nullptr;
     unsigned long long *check = nullptr;  // lym_check_links: max deviation bits, non-finite links
     double *predrawn = nullptr;
@@ -318,6 +320,7 @@ int lym_destroy(lym_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graphs(c);
+    c->bounce.release();
     for (void *p : {c->W[0], c->W[1], c->acc, (void *)c->partials, (void *)c->obs, (void *)c->staging,
                     (void *)c->accepted, (void *)c->check, (void *)c->predrawn, (void *)c->d_order,
                     (void *)c->d_morder})
@@ -348,7 +351,7 @@ int lym_upload_links(lym_ctx *c, const double *const by_mu[4])
     LYM_HIP(c, hipSetDevice(c->device));
     const long V = c->g.V;
     for (int mu = 0; mu < 4; mu++) {
-        LYM_HIP(c, hipMemcpyAsync(c->staging, by_mu[mu], sizeof(double) * 18 * V, hipMemcpyHostToDevice, c->stream));
+        LYM_HIP(c, lym::h2d_bounced(c->bounce, c->stream, c->staging, by_mu[mu], sizeof(double) * 18 * V));
         if (c->prec == LYM_PREC_FP64)
             lym::k_aos_to_soa<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
                 c->staging, static_cast<double2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
@@ -378,8 +381,7 @@ int lym_download_links(lym_ctx *c, double *const by_mu[4])
             lym::k_soa_to_aos<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
                 static_cast<const float2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, c->staging, V);
         LYM_LAUNCHED(c, "k_soa_to_aos");
-        LYM_HIP(c, hipMemcpyAsync(by_mu[mu], c->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, c->stream));
-        LYM_HIP(c, hipStreamSynchronize(c->stream));
+        LYM_HIP(c, lym::d2h_bounced(c->bounce, c->stream, by_mu[mu], c->staging, sizeof(double) * 18 * V));
     }
     return LYM_OK;
 }
@@ -427,8 +429,7 @@ static int export_force(lym_ctx *c, int mu, int what, double *out)
         lym::k_export_force<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
             static_cast<const float2 *>(c->W[c->cur]), c->g, mu, what, c->staging);
     LYM_LAUNCHED(c, "k_export_force");
-    LYM_HIP(c, hipMemcpyAsync(out, c->staging, sizeof(double) * 18 * V, hipMemcpyDeviceToHost, c->stream));
-    LYM_HIP(c, hipStreamSynchronize(c->stream));
+    LYM_HIP(c, lym::d2h_bounced(c->bounce, c->stream, out, c->staging, sizeof(double) * 18 * V));
     return LYM_OK;
 }
 

@@ -384,3 +384,34 @@ def test_even_odd_order_is_invisible(precision):
         a.set_hot(5)  # overwrites every link: no reorder owed
         b.set_hot(5)
         np.testing.assert_array_equal(a.download(), b.download())
+
+
+def test_upload_download_round_trip_large():
+    """Downloads of more than one 64 MB bounce chunk, copied out by several
+    host threads (32 x 32 x 32 x 16: 75.5 MB per direction, a full chunk and a
+    partial one), return the uploaded links bit for bit, into fresh arrays
+    and twice in a row."""
+    dims = (32, 32, 32, 16)
+    V = int(np.prod(dims))
+    rng = np.random.default_rng(11)
+    links = rng.standard_normal((4, V, 18))
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        np.testing.assert_array_equal(ctx.download(), links)
+        np.testing.assert_array_equal(ctx.download(), links)
+
+
+def test_staples_large_lattice_against_small_path():
+    """lym_staples / lym_derivative on a lattice whose output passes through
+    two bounce chunks (32 x 32 x 32 x 16, 75.5 MB) equal the oracle's at
+    every site."""
+    dims = (32, 32, 32, 16)
+    links = orc.hot_start(dims, 19)
+    with LatticeContext(dims) as ctx:
+        ctx.upload(links)
+        om = ctx.staples(2)
+        z = ctx.derivative(3)
+    want_om = orc.staples(links, dims, 2)
+    want_z = orc.derivative(links, dims, 3)
+    assert np.max(np.abs(om - want_om)) < 1e-12
+    assert np.max(np.abs(z - want_z)) < 1e-12

@@ -8,18 +8,28 @@
 
 namespace LatticeIO {
 
+// Sites per fread / fwrite: the file holds, for each site in lexicographic
+// order, the 4 x 18 doubles of its links (direction-major), so blocks of
+// whole sites keep the layout while moving 4.7 MB per call instead of 576 B.
+constexpr int kIoSites = 8192;
+
 void InputConf::readConf(GluonField &lattice, const char *inputFile)
 {
     FILE *f = fopen(inputFile, "rb");
     if (!f) throw std::runtime_error(std::string("cannot open ") + inputFile);
     const int V = lattice[0].sites;
-    std::vector<double> site(4 * 18);
-    for (int s = 0; s < V; s++) {
-        if (fread(site.data(), sizeof(double), site.size(), f) != site.size()) {
+    std::vector<double> block(size_t(kIoSites) * 4 * 18);  // whole sites, 72 doubles each
+    for (int s0 = 0; s0 < V; s0 += kIoSites) {
+        const int n = std::min(kIoSites, V - s0);
+        if (fread(block.data(), sizeof(double) * 4 * 18, size_t(n), f) != size_t(n)) {
             fclose(f);
             throw std::runtime_error(std::string("short configuration file ") + inputFile);
         }
-        for (int mu = 0; mu < 4; mu++) std::copy(&site[18 * mu], &site[18 * mu] + 18, lattice[mu].at(s).mat.begin());
+        for (int i = 0; i < n; i++)
+            for (int mu = 0; mu < 4; mu++) {
+                const double *src = &block[(size_t(i) * 4 + mu) * 18];
+                std::copy(src, src + 18, lattice[mu].at(s0 + i).mat.begin());
+            }
     }
     fclose(f);
 }
@@ -36,10 +46,14 @@ void OutputConf::writeConf(GluonField &lattice, const char *outputFile)
     FILE *f = fopen(outputFile, "wb");
     if (!f) throw std::runtime_error(std::string("cannot create ") + outputFile);
     const int V = lattice[0].sites;
-    std::vector<double> site(4 * 18);
-    for (int s = 0; s < V; s++) {
-        for (int mu = 0; mu < 4; mu++) std::copy(lattice[mu].at(s).mat.begin(), lattice[mu].at(s).mat.end(), &site[18 * mu]);
-        if (fwrite(site.data(), sizeof(double), site.size(), f) != site.size()) {
+    std::vector<double> block(size_t(kIoSites) * 4 * 18);
+    for (int s0 = 0; s0 < V; s0 += kIoSites) {
+        const int n = std::min(kIoSites, V - s0);
+        for (int i = 0; i < n; i++)
+            for (int mu = 0; mu < 4; mu++)
+                std::copy(lattice[mu].at(s0 + i).mat.begin(), lattice[mu].at(s0 + i).mat.end(),
+                          &block[(size_t(i) * 4 + mu) * 18]);
+        if (fwrite(block.data(), sizeof(double) * 4 * 18, size_t(n), f) != size_t(n)) {
             fclose(f);
             throw std::runtime_error(std::string("write failed: ") + outputFile);
         }

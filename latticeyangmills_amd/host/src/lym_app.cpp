@@ -158,7 +158,7 @@ int main(int argc, char **argv)
         }
         if (cmd == "io-selftest") {
             if (argc < 3) return usage();
-            const std::array<int, 4> dims{{2, 4, 2, 4}};
+            const std::array<int, 4> dims{{8, 8, 8, 20}};  // 10240 sites: more than one I/O block
             GluonField a(dims), b(dims);
             std::mt19937_64 rng(1);
             std::uniform_real_distribution<double> u(-1, 1);

@@ -29,8 +29,8 @@ def run(*args):
 
 def test_io_selftest(tmp_path):
     assert "io-selftest ok" in run("io-selftest", tmp_path)
-    a = read_bin(tmp_path / "selftest.bin", (2, 4, 2, 4))
-    assert a.shape == (4, 64, 18) and np.all(np.abs(a) <= 1)
+    a = read_bin(tmp_path / "selftest.bin", (8, 8, 8, 20))  # 10240 sites: a full I/O block and a partial one
+    assert a.shape == (4, 10240, 18) and np.all(np.abs(a) <= 1)
 
 
 @pytest.mark.gpu

@@ -71,34 +71,92 @@ def timed_flow(ctx, eps, tau, every):
     return series, time.perf_counter() - t0
 
 
-def cpu_baseline(beta, seed, budget_s=15.0):
-    """The oracle (FP64, one thread, reference operation order) on a 16^3 x 32
-    sample of the workload: a GPU-thermalised config of 1/16 the volume, flow
-    steps timed with a wall clock as the reference times its runs."""
-    from latticeyangmills_amd import LatticeContext
+def _oracle():
     so = os.path.join(ROOT, "oracle", "_build", "liblym_oracle.so")
     if not os.path.isfile(so):
         return None
     orc = ctypes.CDLL(so)
     D = ctypes.POINTER(ctypes.c_double)
     orc.or_flow_step.argtypes = [D, ctypes.c_int * 4, ctypes.c_double]
-    dims = (16, 16, 16, 32)
-    with LatticeContext(dims, "fp64") as ctx:
-        ctx.metropolis(beta, 0.24, 10, 20, 0, "exph", seed)
-        links = np.ascontiguousarray(ctx.download())
+    orc.or_observables.argtypes = [D, ctypes.c_int * 4, ctypes.c_int, D]
+    orc.or_hot_start.argtypes = [D, ctypes.c_int * 4, ctypes.c_uint64]
+    return orc
+
+
+def cpu_baseline(links, dims, eps, beta, budget_s=20.0):
+    """The oracle (FP64, one thread, the reference's operation order) on the
+    benchmark's own lattice and configuration: whole flow steps timed with a
+    wall clock, as the reference times its runs; as many steps as fit the
+    budget, at least one."""
+    orc = _oracle()
+    if orc is None:
+        return None
+    D = ctypes.POINTER(ctypes.c_double)
+    work = np.array(links, copy=True)
     V = int(np.prod(dims))
     d = (ctypes.c_int * 4)(*dims)
     steps, t0 = 0, time.perf_counter()
     while True:
-        assert orc.or_flow_step(links.ctypes.data_as(D), d, 0.02) == 0
+        assert orc.or_flow_step(work.ctypes.data_as(D), d, eps) == 0
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 20:
+        if el + el / steps > budget_s:
             break
     return {"value": 4 * V * steps / el, "unit": "link-updates/s", "cores": 1, "kind": "port",
-            "sample": f"oracle FP64 RK3 flow, {steps} steps of eps=0.02 on a GPU-thermalised 16^3x32 beta={beta} "
-                      f"config ({el:.1f} s wall); per-link-update cost is volume-independent at this size",
+            "sample": f"oracle FP64 RK3 flow, {steps} step(s) of eps={eps} on the benchmark's GPU-thermalised "
+                      f"{'x'.join(map(str, dims))} beta={beta} configuration ({el:.1f} s wall)",
             "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+
+
+def _ensemble_cpu_worker(args):
+    """One process of the ensemble CPU leg: oracle flow steps and one
+    measurement on a Haar-random lattice of the given shape, timed."""
+    dims, seed, nsteps, eps = args
+    orc = _oracle()
+    D = ctypes.POINTER(ctypes.c_double)
+    V = int(np.prod(dims))
+    links = np.empty((4, V, 18))
+    d = (ctypes.c_int * 4)(*dims)
+    orc.or_hot_start(links.ctypes.data_as(D), d, seed)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        assert orc.or_flow_step(links.ctypes.data_as(D), d, eps) == 0
+    t1 = time.perf_counter()
+    out = np.empty(4)
+    orc.or_observables(links.ctypes.data_as(D), d, 0, out.ctypes.data_as(D))
+    t2 = time.perf_counter()
+    return tuple(dims), (t1 - t0) / nsteps, t2 - t1
+
+
+def cpu_ensemble_baseline(eps=0.01, tau=2.0, every=10, max_procs=16):
+    """The ensemble's CPU leg: P single-thread oracle processes at once (P = the
+    cores this process may use, at most max_procs), one configuration each, the
+    four lattice shapes of the ensemble in turn.  Each process times two flow
+    steps and one measurement of its lattice; a whole configuration costs
+    (int(tau/eps) - 1) steps and one measurement per `every` steps, so P
+    processes flow P / (mean configuration time) configurations per second.
+    Run before the GPU is touched (the workers are forked)."""
+    import multiprocessing as mp
+    if _oracle() is None:
+        return None
+    procs = max(1, min(len(os.sched_getaffinity(0)), max_procs))
+    shapes = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 32)]
+    jobs = [(shapes[i % 4], i, 2, eps) for i in range(max(procs, 4))]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_ensemble_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    steps = int(tau / eps) - 1
+    nmeas = 1 + steps // every
+    per_shape = {}
+    for dims, t_step, t_obs in res:
+        per_shape.setdefault(dims, []).append(steps * t_step + nmeas * t_obs)
+    conf_s = float(np.mean([np.mean(v) for v in per_shape.values()]))
+    return {"value": procs / conf_s, "unit": "configs/s", "cores": procs, "kind": "port",
+            "sample": f"{len(jobs)} oracle processes ({procs} at once), each 2 flow steps + 1 measurement of one "
+                      f"Haar-random lattice (8^4, 12^4, 16^4, 16^3x32 in turn; {wall:.1f} s wall); a configuration "
+                      f"= {steps} steps + {nmeas} measurements, mean {conf_s:.1f} s",
+            "seconds_per_config": {"x".join(map(str, k)): float(np.mean(v)) for k, v in per_shape.items()}}
 
 
 BATCH_DIMS = [(8, 8, 8, 8)] * 16 + [(12, 12, 12, 12)] * 16 + [(16, 16, 16, 16)] * 16 + [(16, 16, 16, 32)] * 16
@@ -175,12 +233,19 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    # the ensemble's CPU leg forks its worker processes, so it runs before
+    # anything touches the GPU
+    cpu_ens = None if (args.no_cpu or args.no_batch) else cpu_ensemble_baseline()
+    if cpu_ens:
+        log(f"[bench] CPU ensemble leg: {cpu_ens['value']:.4f} configs/s on {cpu_ens['cores']} processes")
+
     from latticeyangmills_amd import LatticeContext
     w = WORKLOADS[args.workload]
     dims, eps, every = w["dims"], w["eps"], w["every"]
     V = int(np.prod(dims))
-    steps = args.steps if args.steps else int(w["tau"] / eps) - 1
-    tau = (steps + 1) * eps  # reference semantics: int(tau/eps) - 1 steps
+    full_steps = int(w["tau"] / eps) - 1  # reference semantics: int(tau/eps) - 1 steps
+    steps = args.steps if args.steps else full_steps
+    tau = (steps + 1) * eps
     log(f"[bench] workload {args.workload} dims={dims} V={V} steps={steps} warmup={args.warmup}")
 
     # thermalise in FP64 on the GPU
@@ -214,6 +279,11 @@ def main():
                 ctx.synchronize()
                 ctx.upload(links)
             series, el = timed_flow(ctx, eps, tau, every)
+            if steps != full_steps:
+                # the error statement always covers the whole flow to t = tau - eps,
+                # whatever the timed step count (untimed)
+                ctx.upload(links)
+                series = ctx.flow(eps, w["tau"], every)
             ctx.upload(links)
             total_ms, st = ctx.profile_flow(eps, args.profile_steps, every)
         results[prec] = dict(series=series, seconds=el, stage_ms=st, ms_per_step=1e3 * el / steps, tuner=tuner,
@@ -269,6 +339,7 @@ def main():
         out["fp64"] = {"value": lups64, "ms_per_step": f["ms_per_step"], "speedup_mixed": lups / lups64,
                        "stage_ms": [float(x) for x in f["stage_ms"][:3]], "schedule": f["tuner"]}
         out["accuracy_vs_fp64"] = {
+            "flow": f"whole flow, eps {eps} to t = {s64[last, 0]:.2f} ({len(s64) - 1} steps), every {every}",
             "max_rel_err_E": e_err, "max_abs_err_Q": q_err,
             "t0_a2_mixed": t0m, "t0_a2_fp64": t064,
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),
@@ -279,12 +350,14 @@ def main():
         }
     if not args.no_batch:
         out["batched"] = batched_ensemble(args.tune_cache)
+        if cpu_ens:
+            out["batched"]["cpu_baseline"] = cpu_ens
         log(f"[bench] batched: {out['batched']}")
     if args.calibrate:
         from latticeyangmills_amd.capi import copy_bandwidth
         out["hbm_copy_gbs"] = copy_bandwidth(1 << 30, 10)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w["beta"], w["seed"])
+        out["cpu_baseline"] = cpu_baseline(links, dims, eps, w["beta"])
     print(json.dumps(out), flush=True)
 
 

@@ -4,6 +4,7 @@ liblym.so (HIP kernels + the C ABI of include/lym.h) and liblymhost.so (the
 C++ host classes) land next to this file; the CPU oracle is built by
 oracle/Makefile.  Run as ``python -m latticeyangmills_amd.build``.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -21,7 +22,12 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["lym_capi.hip", "lym_batch.hip"]
+# One object per translation unit, compiled in parallel: the stage and
+# Metropolis kernel families are instantiated per link precision in units of
+# their own (lym_launch.hpp), the C ABI of the two paths in lym_capi / lym_batch.
+LIB_SOURCES = ["lym_bstage_f64.hip", "lym_bstage_f32.hip", "lym_stage_f64.hip", "lym_stage_f32.hip",
+               "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip", "lym_batch.hip"]
+OBJ_DIR = os.path.join(PKG, "_build")
 
 
 def _run(cmd, cwd=None):
@@ -29,10 +35,24 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+def _jobs():
+    n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get("MAX_JOBS", n)), len(LIB_SOURCES)))
+
+
 def build_lym(extra=()):
     out = os.path.join(PKG, "liblym.so")
-    srcs = [os.path.join(CSRC, s) for s in LIB_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, *extra, "-shared", "-I", INCLUDE, "-I", CSRC, "-o", out + ".tmp", *srcs])
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in LIB_SOURCES]
+
+    def compile_one(i):
+        _run([HIPCC, *HIP_FLAGS, *extra, "-c", "-I", INCLUDE, "-I", CSRC, "-o", objs[i],
+              os.path.join(CSRC, LIB_SOURCES[i])])
+
+    with concurrent.futures.ThreadPoolExecutor(_jobs()) as pool:
+        for f in [pool.submit(compile_one, i) for i in range(len(LIB_SOURCES))]:
+            f.result()
+    _run([HIPCC, *HIP_FLAGS, *extra, "-shared", "-o", out + ".tmp", *objs])
     os.replace(out + ".tmp", out)
     return out
 

@@ -7,7 +7,7 @@
 #include <vector>
 
 #include "lym.h"
-#include "lym_kernels.hpp"
+#include "lym_launch.hpp"
 #include "lym_sched.hpp"
 #include "lym_xfer.hpp"
 
@@ -168,22 +168,24 @@ int apply_schedule(lym_ctx *c, int kind, int target)
     return LYM_OK;
 }
 
+// LDS staging mode of the stage kernel for this lattice under the schedule
+// (flow_block): slab where whole x-rows never cross a z-plane, whole x-rows,
+// 16 x 4 tiles of every (z, t) plane, else the run's own links.
+int stage_lds_mode(const lym_ctx *c)
+{
+    if (!(c->sched.kind & lym::SCHED_LDS)) return 0;
+    if (lym::slab_fits(c->dims)) return 3;
+    if (lym::kSitesPerBlock % c->dims[0] == 0) return 2;
+    if (lym::tile_fits(c->dims)) return 5;
+    return 1;
+}
+
 // one RK stage over the lattice with the context's schedule and kernel variant
 template <typename S, int STAGE>
 void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *acc, double eps)
 {
-    constexpr bool reunit = sizeof(S) == 4;
-    const unsigned nb = stencil_blocks(c);
-    if (!(c->sched.kind & lym::SCHED_LDS))
-        lym::k_flow_stage<S, STAGE, reunit, 0><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
-    else if (lym::slab_fits(c->dims))  // whole x-rows, never crossing a z-plane
-        lym::k_flow_stage<S, STAGE, reunit, 3><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
-    else if (lym::kSitesPerBlock % c->dims[0] == 0)  // runs hold whole x-rows
-        lym::k_flow_stage<S, STAGE, reunit, 2><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
-    else if (lym::tile_fits(c->dims))  // 16 x 4 blocks that tile every (z, t) plane
-        lym::k_flow_stage<S, STAGE, reunit, 5><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
-    else
-        lym::k_flow_stage<S, STAGE, reunit, 1><<<nb, 256, 0, c->stream>>>(win, wout, acc, c->g, dev_sched(c), eps);
+    lym::launch_flow_stage<S, STAGE>(stage_lds_mode(c), stencil_blocks(c), c->stream, win, wout, acc, c->g,
+                                     dev_sched(c), eps);
 }
 
 template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
@@ -548,24 +550,9 @@ extern "C++" template <typename S>
 static void launch_metropolis(lym_ctx *c, unsigned nb, int mu, const lym::MetroParams &p)
 {
     const unsigned per_xcd = c->d_morder ? unsigned(c->morder_per_xcd) : (nb + 7) / 8;
-    const int *ord = c->d_morder;
-    auto *f = static_cast<lym::F2<S> *>(c->W[c->cur]);
-    if (p.candidate == LYM_CAND_SU2) {
-        if (p.predrawn)
-            lym::k_metropolis<S, 0, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-        else
-            lym::k_metropolis<S, 0, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-    } else if (8.0 * p.eps * p.eps > lym::kSeriesC1Max) {
-        if (p.predrawn)
-            lym::k_metropolis<S, 1, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-        else
-            lym::k_metropolis<S, 1, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-    } else {  // every candidate has c1 < 8 eps^2: series-only exponential
-        if (p.predrawn)
-            lym::k_metropolis<S, 2, true><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-        else
-            lym::k_metropolis<S, 2, false><<<8 * per_xcd, 256, 0, c->stream>>>(f, c->g, mu, p, ord, per_xcd, c->accepted);
-    }
+    lym::launch_metropolis<S>(lym::metro_kernel_kind(p.candidate == LYM_CAND_EXPH, p.eps), p.predrawn != nullptr,
+                              per_xcd, c->stream, static_cast<lym::F2<S> *>(c->W[c->cur]), c->g, mu, p, c->d_morder,
+                              c->accepted);
 }
 
 int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, int first_sweep, int candidate,

@@ -479,34 +479,38 @@ constexpr double kSeriesC1Max = 0.5;
 constexpr int kSeriesTerms = 18;
 // Short tier: c1 <= 0.004 (spectral radius <= 0.073) needs 9 terms for a
 // truncation error below 2e-18.  The Wilson flow at eps <= 0.02 is almost
-// always there; the tier is chosen per wave (all lanes qualify) so that the
-// wave never runs both loops.
+// always there.  Each lane decides from its own c1: terms 0..9 run for every
+// lane, terms 10..18 only for lanes above the tier (a wave whose lanes are all
+// in the tier skips them), so a link's result depends on its own Q alone and
+// not on which sites share its wave.
 constexpr double kShortSeriesC1Max = 0.004;
 constexpr int kShortSeriesTerms = 9;
 
-template <int N>
-LYM_DI void series_coeffs(double c0, double c1, cplx<double> &f0, cplx<double> &f1, cplx<double> &f2)
+// Running state of the series: Q^n = a + b Q + d Q^2, inv_fact = 1/n!, and the
+// partial sums of f0, f1, f2 over the terms added so far.
+struct SeriesState {
+    double a, b, d, inv_fact;
+    double f0r, f0i, f1r, f1i, f2r, f2i;
+};
+
+// adds the terms n = N0 .. N1 (in order) to the sums
+template <int N0, int N1> LYM_DI void series_terms(SeriesState &st, double c0, double c1)
 {
-    double a = 1.0, b = 0.0, d = 0.0, inv_fact = 1.0;
-    double f0r = 0, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
 #pragma unroll
-    for (int n = 0; n <= N; n++) {
-        if (n > 0) inv_fact *= 1.0 / double(n);
-        const double ta = a * inv_fact, tb = b * inv_fact, td = d * inv_fact;
+    for (int n = N0; n <= N1; n++) {
+        if (n > 0) st.inv_fact *= 1.0 / double(n);
+        const double ta = st.a * st.inv_fact, tb = st.b * st.inv_fact, td = st.d * st.inv_fact;
         switch (n & 3) {  // i^n = 1, i, -1, -i
-        case 0: f0r += ta; f1r += tb; f2r += td; break;
-        case 1: f0i += ta; f1i += tb; f2i += td; break;
-        case 2: f0r -= ta; f1r -= tb; f2r -= td; break;
-        default: f0i -= ta; f1i -= tb; f2i -= td; break;
+        case 0: st.f0r += ta; st.f1r += tb; st.f2r += td; break;
+        case 1: st.f0i += ta; st.f1i += tb; st.f2i += td; break;
+        case 2: st.f0r -= ta; st.f1r -= tb; st.f2r -= td; break;
+        default: st.f0i -= ta; st.f1i -= tb; st.f2i -= td; break;
         }
-        const double an = c0 * d, bn = fma(c1, d, a);
-        d = b;
-        a = an;
-        b = bn;
+        const double an = c0 * st.d, bn = fma(c1, st.d, st.a);
+        st.d = st.b;
+        st.a = an;
+        st.b = bn;
     }
-    f0 = cplx<double>{f0r, f0i};
-    f1 = cplx<double>{f1r, f1i};
-    f2 = cplx<double>{f2r, f2i};
 }
 
 LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, double d2, double c1)
@@ -517,12 +521,11 @@ LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, 
     const double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
     const double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
     const double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
-    cplx<double> f0, f1, f2;
-    if (__all(c1 <= kShortSeriesC1Max))
-        series_coeffs<kShortSeriesTerms>(c0, c1, f0, f1, f2);
-    else
-        series_coeffs<kSeriesTerms>(c0, c1, f0, f1, f2);
-    return exp_assemble(q, q2, d2, f0, f1, f2);
+    SeriesState st{1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    series_terms<0, kShortSeriesTerms>(st, c0, c1);
+    if (c1 > kShortSeriesC1Max) series_terms<kShortSeriesTerms + 1, kSeriesTerms>(st, c0, c1);
+    return exp_assemble(q, q2, d2, cplx<double>{st.f0r, st.f0i}, cplx<double>{st.f1r, st.f1i},
+                        cplx<double>{st.f2r, st.f2i});
 }
 
 // exp(iQ) by the series alone, for Q known to have c1 = Tr Q^2 / 2 <= kSeriesC1Max

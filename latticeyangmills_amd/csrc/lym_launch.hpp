@@ -1,0 +1,107 @@
+// Host-side launchers of the heavy kernel families: the RK stage kernels of
+// the single-lattice and batch paths and the Metropolis sub-sweep kernels.
+// Each family is instantiated for one link precision per translation unit
+// (lym_stage_*.hip: single lattice, lym_bstage_*.hip: batch,
+// lym_metro_*.hip: Metropolis; f64 and f32), so the device code of the
+// library compiles in parallel; every other translation unit sees the extern
+// declarations below.
+#pragma once
+#include "lym_batch_kernels.hpp"
+
+namespace lym {
+
+// One RK stage over one lattice: nb workgroups of 256 threads on stream st;
+// lds = the flow_block staging mode (0, 1, 2, 3 or 5).
+template <typename S, int STAGE>
+void launch_flow_stage(int lds, unsigned nb, hipStream_t st, const F2<S> *win, F2<S> *wout, S *acc, Geom g, Sched sc,
+                       double eps)
+{
+    constexpr bool re = sizeof(S) == 4;  // mixed path: reunitarise before the store
+    switch (lds) {
+    case 0: k_flow_stage<S, STAGE, re, 0><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 1: k_flow_stage<S, STAGE, re, 1><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 2: k_flow_stage<S, STAGE, re, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 3: k_flow_stage<S, STAGE, re, 3><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    default: k_flow_stage<S, STAGE, re, 5><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    }
+}
+
+// One RK stage over every lattice of a batch: grid workgroups stride over the
+// schedule slots; lds = 0, 1, 2, 3 or 4 (per-lattice slab or own links).
+template <typename S, int STAGE>
+void launch_batch_stage(int lds, unsigned grid, hipStream_t st, const F2<S> *win, F2<S> *wout, S *acc,
+                        const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps)
+{
+    constexpr bool re = sizeof(S) == 4;
+    switch (lds) {
+    case 0: k_batch_flow_stage<S, STAGE, re, 0><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 1: k_batch_flow_stage<S, STAGE, re, 1><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 2: k_batch_flow_stage<S, STAGE, re, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 3: k_batch_flow_stage<S, STAGE, re, 3><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    default: k_batch_flow_stage<S, STAGE, re, 4><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    }
+}
+
+// Candidate kind of a Metropolis launch: 0 = R S T product, 1 = exp(i eps H)
+// with the general exponential, 2 = exp(i eps H) with 8 eps^2 <= kSeriesC1Max
+// (series-only exponential).
+inline int metro_kernel_kind(int candidate_is_exph, double eps)
+{
+    if (!candidate_is_exph) return 0;
+    return 8.0 * eps * eps > kSeriesC1Max ? 1 : 2;
+}
+
+// One Metropolis sub-sweep over one lattice in even-odd order (k_metropolis).
+template <typename S>
+void launch_metropolis(int kind, bool predrawn, unsigned per_xcd, hipStream_t st, F2<S> *f, Geom g, int mu,
+                       const MetroParams &p, const int *order, unsigned long long *accepted)
+{
+    const unsigned nb = 8 * per_xcd;
+    if (kind == 0) {
+        if (predrawn) k_metropolis<S, 0, true><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+        else k_metropolis<S, 0, false><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+    } else if (kind == 1) {
+        if (predrawn) k_metropolis<S, 1, true><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+        else k_metropolis<S, 1, false><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+    } else {
+        if (predrawn) k_metropolis<S, 2, true><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+        else k_metropolis<S, 2, false><<<nb, 256, 0, st>>>(f, g, mu, p, order, per_xcd, accepted);
+    }
+}
+
+// One Metropolis sub-sweep over every lattice of a batch (k_batch_metropolis).
+template <typename S>
+void launch_batch_metropolis(int kind, unsigned per_xcd, hipStream_t st, F2<S> *f, const ConfDesc *desc, int nconf,
+                             long nitems, int mu, const MetroParams &p, unsigned long long *accepted)
+{
+    const unsigned nb = 8 * per_xcd;
+    if (kind == 0) k_batch_metropolis<S, 0><<<nb, 256, 0, st>>>(f, desc, nconf, nitems, mu, p, per_xcd, accepted);
+    else if (kind == 1) k_batch_metropolis<S, 1><<<nb, 256, 0, st>>>(f, desc, nconf, nitems, mu, p, per_xcd, accepted);
+    else k_batch_metropolis<S, 2><<<nb, 256, 0, st>>>(f, desc, nconf, nitems, mu, p, per_xcd, accepted);
+}
+
+#define LYM_STAGE_EXTERN(S, STAGE)                                                                                 \
+    extern template void launch_flow_stage<S, STAGE>(int, unsigned, hipStream_t, const F2<S> *, F2<S> *, S *, Geom, \
+                                                     Sched, double);                                               \
+    extern template void launch_batch_stage<S, STAGE>(int, unsigned, hipStream_t, const F2<S> *, F2<S> *, S *,    \
+                                                      const ConfDesc *, int, long, Sched, double);
+#define LYM_METRO_EXTERN(S)                                                                                        \
+    extern template void launch_metropolis<S>(int, bool, unsigned, hipStream_t, F2<S> *, Geom, int,               \
+                                              const MetroParams &, const int *, unsigned long long *);             \
+    extern template void launch_batch_metropolis<S>(int, unsigned, hipStream_t, F2<S> *, const ConfDesc *, int,  \
+                                                    long, int, const MetroParams &, unsigned long long *);
+
+#ifndef LYM_INSTANTIATE_STAGES
+LYM_STAGE_EXTERN(double, 1)
+LYM_STAGE_EXTERN(double, 2)
+LYM_STAGE_EXTERN(double, 3)
+LYM_STAGE_EXTERN(float, 1)
+LYM_STAGE_EXTERN(float, 2)
+LYM_STAGE_EXTERN(float, 3)
+#endif
+#ifndef LYM_INSTANTIATE_METRO
+LYM_METRO_EXTERN(double)
+LYM_METRO_EXTERN(float)
+#endif
+
+}  // namespace lym

@@ -310,7 +310,7 @@ def main():
                       "note": "one upload and one download of the configuration (GluonField layout in host memory) "
                               "around the timed flow; value excludes them"},
         "config": {"workload": args.workload, "lattice": "x".join(map(str, dims)), "beta": w["beta"],
-                   "start": w.get("start", "cold"), "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "flow_t_max": tau - eps,
+                   "start": w.get("start", "cold"), "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "timed_flow_t_max": tau - eps,
                    "measure_every": every, "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -339,7 +339,8 @@ def main():
         out["fp64"] = {"value": lups64, "ms_per_step": f["ms_per_step"], "speedup_mixed": lups / lups64,
                        "stage_ms": [float(x) for x in f["stage_ms"][:3]], "schedule": f["tuner"]}
         out["accuracy_vs_fp64"] = {
-            "flow": f"whole flow, eps {eps} to t = {s64[last, 0]:.2f} ({len(s64) - 1} steps), every {every}",
+            "flow": f"whole flow, eps {eps} to t = {s64[-1, 0]:.2f} ({len(s64) - 1} steps), measured every {every} "
+                    f"steps (last measured row t = {s64[last, 0]:.2f})",
             "max_rel_err_E": e_err, "max_abs_err_Q": q_err,
             "t0_a2_mixed": t0m, "t0_a2_fp64": t064,
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),

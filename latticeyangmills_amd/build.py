@@ -15,9 +15,14 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-# Every HIP translation unit is built without packed FP32 instructions.
+# Every HIP translation unit is built without packed FP32 instructions, and
+# with multiply-adds fused only where one source expression asks for it
+# (-ffp-contract=on; the hot code spells its fusions out with fma()).  The
+# default for HIP fuses across statements inside a basic block, so kernel
+# variants that differ only in where a link is read from (LDS or HBM, other
+# branches and barriers) fused different pairs and rounded differently.
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

@@ -425,13 +425,13 @@ LYM_DI Nbr neighbours(const Geom g, long s)
 LYM_DI su3<double> herm_square(const herm8<double> &q)
 {
     double d2 = -q.d0 - q.d1;
-    double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
-    double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
-    double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
+    double n01 = fma(q.o01.re, q.o01.re, q.o01.im * q.o01.im);
+    double n02 = fma(q.o02.re, q.o02.re, q.o02.im * q.o02.im);
+    double n12 = fma(q.o12.re, q.o12.re, q.o12.im * q.o12.im);
     su3<double> r;
-    r.e[0] = cplx<double>{q.d0 * q.d0 + n01 + n02, 0.0};
-    r.e[4] = cplx<double>{q.d1 * q.d1 + n01 + n12, 0.0};
-    r.e[8] = cplx<double>{d2 * d2 + n02 + n12, 0.0};
+    r.e[0] = cplx<double>{fma(q.d0, q.d0, n01 + n02), 0.0};
+    r.e[4] = cplx<double>{fma(q.d1, q.d1, n01 + n12), 0.0};
+    r.e[8] = cplx<double>{fma(d2, d2, n02 + n12), 0.0};
     // (Q^2)_01 = (d0 + d1) o01 + o02 conj(o12)
     cplx<double> t01{(q.d0 + q.d1) * q.o01.re, (q.d0 + q.d1) * q.o01.im};
     t01 = cfma_nc(q.o02, q.o12, t01);
@@ -455,9 +455,12 @@ LYM_DI su3<double> exp_assemble(const herm8<double> &q, const su3<double> &q2, d
                                 cplx<double> f1, cplx<double> f2)
 {
     su3<double> r;
-    r.e[0] = cplx<double>{f0.re + f1.re * q.d0 + f2.re * q2.e[0].re, f0.im + f1.im * q.d0 + f2.im * q2.e[0].re};
-    r.e[4] = cplx<double>{f0.re + f1.re * q.d1 + f2.re * q2.e[4].re, f0.im + f1.im * q.d1 + f2.im * q2.e[4].re};
-    r.e[8] = cplx<double>{f0.re + f1.re * d2 + f2.re * q2.e[8].re, f0.im + f1.im * d2 + f2.im * q2.e[8].re};
+    const double dg[3] = {q.d0, q.d1, d2};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double q2ii = q2.e[4 * i].re;
+        r.e[4 * i] = cplx<double>{fma(f2.re, q2ii, fma(f1.re, dg[i], f0.re)), fma(f2.im, q2ii, fma(f1.im, dg[i], f0.im))};
+    }
     r.e[1] = cfma(f2, q2.e[1], cfma(f1, q.o01, cplx<double>{0.0, 0.0}));
     r.e[2] = cfma(f2, q2.e[2], cfma(f1, q.o02, cplx<double>{0.0, 0.0}));
     r.e[5] = cfma(f2, q2.e[5], cfma(f1, q.o12, cplx<double>{0.0, 0.0}));
@@ -499,12 +502,16 @@ template <int N0, int N1> LYM_DI void series_terms(SeriesState &st, double c0, d
 #pragma unroll
     for (int n = N0; n <= N1; n++) {
         if (n > 0) st.inv_fact *= 1.0 / double(n);
-        const double ta = st.a * st.inv_fact, tb = st.b * st.inv_fact, td = st.d * st.inv_fact;
-        switch (n & 3) {  // i^n = 1, i, -1, -i
-        case 0: st.f0r += ta; st.f1r += tb; st.f2r += td; break;
-        case 1: st.f0i += ta; st.f1i += tb; st.f2i += td; break;
-        case 2: st.f0r -= ta; st.f1r -= tb; st.f2r -= td; break;
-        default: st.f0i -= ta; st.f1i -= tb; st.f2i -= td; break;
+        const double sg = (n & 2) ? -1.0 : 1.0;  // i^n = 1, i, -1, -i: real for even n
+        const double w = sg * st.inv_fact;
+        if (n & 1) {
+            st.f0i = fma(st.a, w, st.f0i);
+            st.f1i = fma(st.b, w, st.f1i);
+            st.f2i = fma(st.d, w, st.f2i);
+        } else {
+            st.f0r = fma(st.a, w, st.f0r);
+            st.f1r = fma(st.b, w, st.f1r);
+            st.f2r = fma(st.d, w, st.f2r);
         }
         const double an = c0 * st.d, bn = fma(c1, st.d, st.a);
         st.d = st.b;
@@ -516,11 +523,12 @@ template <int N0, int N1> LYM_DI void series_terms(SeriesState &st, double c0, d
 LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, double d2, double c1)
 {
     cplx<double> p = cfma(q.o01, q.o12, cplx<double>{0.0, 0.0});
-    const double re3 = p.re * q.o02.re + p.im * q.o02.im;
-    const double n01 = q.o01.re * q.o01.re + q.o01.im * q.o01.im;
-    const double n02 = q.o02.re * q.o02.re + q.o02.im * q.o02.im;
-    const double n12 = q.o12.re * q.o12.re + q.o12.im * q.o12.im;
-    const double c0 = q.d0 * q.d1 * d2 + 2.0 * re3 - q.d0 * n12 - q.d1 * n02 - d2 * n01;
+    const double re3 = fma(p.re, q.o02.re, p.im * q.o02.im);
+    const double n01 = fma(q.o01.re, q.o01.re, q.o01.im * q.o01.im);
+    const double n02 = fma(q.o02.re, q.o02.re, q.o02.im * q.o02.im);
+    const double n12 = fma(q.o12.re, q.o12.re, q.o12.im * q.o12.im);
+    // c0 = det Q = d0 d1 d2 + 2 Re(o01 o12 conj(o02)) - d0 |o12|^2 - d1 |o02|^2 - d2 |o01|^2
+    const double c0 = fma(-d2, n01, fma(-q.d1, n02, fma(-q.d0, n12, fma(q.d0 * q.d1, d2, 2.0 * re3))));
     SeriesState st{1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     series_terms<0, kShortSeriesTerms>(st, c0, c1);
     if (c1 > kShortSeriesC1Max) series_terms<kShortSeriesTerms + 1, kSeriesTerms>(st, c0, c1);
@@ -676,33 +684,30 @@ LYM_DI void reunitarize(su3<double> &m)
 {
     double n0 = 0.0;
 #pragma unroll
-    for (int j = 0; j < 3; j++) n0 += m.e[j].re * m.e[j].re + m.e[j].im * m.e[j].im;
+    for (int j = 0; j < 3; j++) n0 = fma(m.e[j].im, m.e[j].im, fma(m.e[j].re, m.e[j].re, n0));
     double r0 = rsqrt(n0);
-    r0 = r0 * (1.5 - 0.5 * n0 * r0 * r0);  // one Newton step for full FP64 accuracy
+    r0 = r0 * fma(-0.5 * n0, r0 * r0, 1.5);  // one Newton step for full FP64 accuracy
 #pragma unroll
     for (int j = 0; j < 3; j++) m.e[j] = cplx<double>{m.e[j].re * r0, m.e[j].im * r0};
     cplx<double> p{0.0, 0.0};  // <row0|row1>
 #pragma unroll
     for (int j = 0; j < 3; j++) p = cfma_cn(m.e[j], m.e[3 + j], p);
+    const cplx<double> mp{-p.re, -p.im};
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        cplx<double> t = cfma(p, m.e[j], cplx<double>{0.0, 0.0});
-        m.e[3 + j] = cplx<double>{m.e[3 + j].re - t.re, m.e[3 + j].im - t.im};
-    }
+    for (int j = 0; j < 3; j++) m.e[3 + j] = cfma(mp, m.e[j], m.e[3 + j]);  // row1 -= <row0|row1> row0
     double n1 = 0.0;
 #pragma unroll
-    for (int j = 0; j < 3; j++) n1 += m.e[3 + j].re * m.e[3 + j].re + m.e[3 + j].im * m.e[3 + j].im;
+    for (int j = 0; j < 3; j++) n1 = fma(m.e[3 + j].im, m.e[3 + j].im, fma(m.e[3 + j].re, m.e[3 + j].re, n1));
     double r1 = rsqrt(n1);
-    r1 = r1 * (1.5 - 0.5 * n1 * r1 * r1);
+    r1 = r1 * fma(-0.5 * n1, r1 * r1, 1.5);
 #pragma unroll
     for (int j = 0; j < 3; j++) m.e[3 + j] = cplx<double>{m.e[3 + j].re * r1, m.e[3 + j].im * r1};
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-        int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    for (int j = 0; j < 3; j++) {  // row2 = conj(row0 x row1)
+        const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        const cplx<double> a = m.e[j2];
         cplx<double> c = cfma(m.e[j1], m.e[3 + j2], cplx<double>{0.0, 0.0});
-        cplx<double> a = m.e[j2], b = m.e[3 + j1];
-        c.re -= a.re * b.re - a.im * b.im;
-        c.im -= a.re * b.im + a.im * b.re;
+        c = cfma(cplx<double>{-a.re, -a.im}, m.e[3 + j1], c);
         m.e[6 + j] = conj(c);
     }
 }

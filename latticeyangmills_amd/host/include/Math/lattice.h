@@ -68,5 +68,28 @@ template <typename T> T sum(const Lattice<T> &a)
     return r;
 }
 
+// Site-wise exp(iQ) of a lattice of Hermitian traceless matrices
+// (include/Math/lattice.h:231-245 of the reference).
+template <typename T> Lattice<T> exp(const Lattice<T> &a)
+{
+    Lattice<T> r(a.size);
+    for (int i = 0; i < a.sites; i++) r.at(i) = exp(a.at(i));
+    return r;
+}
+
+// Site-wise Re Tr and Im Tr (include/Math/lattice.h:298-324 of the reference).
+inline Lattice<double> realTrace(const Lattice<SU3> &a)
+{
+    Lattice<double> r(a.size);
+    for (int i = 0; i < a.sites; i++) r.at(i) = a.at(i).realTrace();
+    return r;
+}
+inline Lattice<double> imagTrace(const Lattice<SU3> &a)
+{
+    Lattice<double> r(a.size);
+    for (int i = 0; i < a.sites; i++) r.at(i) = a.at(i).imagTrace();
+    return r;
+}
+
 void setToZero(Lattice<SU3> &l);
 void setToIdentity(Lattice<SU3> &l);

@@ -186,7 +186,7 @@ def test_graph_is_recaptured_for_a_new_eps(inputs):
     """The captured flow graph bakes eps into its launches: a step size that
     differs in the last bit (same row count) must capture a new graph, so the
     replay equals an eager run at that eps."""
-    eps0, tau = 0.02, 0.1
+    eps0, tau = 0.02, 0.11
     eps1 = float(np.nextafter(eps0, 1.0))
     assert int(tau / eps0) == int(tau / eps1)
     with BatchContext(DIMS, "fp64") as b, BatchContext(DIMS, "fp64") as e:

@@ -63,7 +63,7 @@ def build_lym(extra=()):
 
 
 HOST = os.path.join(PKG, "host")
-HOST_SOURCES = ["math.cpp", "gpuclasses.cpp", "confio.cpp", "jsonlite.cpp", "scale.cpp"]
+HOST_SOURCES = ["math.cpp", "devicefield.cpp", "gpuclasses.cpp", "confio.cpp", "jsonlite.cpp", "scale.cpp"]
 CXX = os.environ.get("CXX", "g++")
 CXX_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-result"]
 

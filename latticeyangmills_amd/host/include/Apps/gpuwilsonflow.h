@@ -24,10 +24,32 @@ public:
     void setOutputDir(const std::string &d) { m_outputDir = d; }
     std::vector<std::string> &inputList() { return m_inputConfList; }
 
+    // getters of the reference (include/Apps/wilsonflow.h:58-60); the values
+    // are those of the observables at the last measured row of the last flow
+    std::array<int, 4> &getSize() { return m_size; }
+    std::vector<double> &getObsValues() { return m_obsValues; }
+    std::vector<Observable *> &getObs() { return m_obs; }
+
+    /// on (default): flowConfigurations flows the input files in batches, all
+    /// of a batch in one lym_batch_flow call (include/lym_batch.h), up to
+    /// batchBytes of device memory per batch; off: one after another through
+    /// the single-lattice context, as the reference's loop
+    void setBatchFlow(bool on, double batchBytes = 32.0 * (1ull << 30))
+    {
+        m_batchFlow = on;
+        m_batchBytes = batchBytes;
+    }
+
 private:
     void computeObservables();
+    void flowBatch(size_t first, size_t last);
+    /// output of one flowed configuration: scales, observables, flow/conf%04d.dat
+    void finishConfiguration(int confNum, const std::vector<std::array<double, 4>> &series);
     double m_tauFinal, m_epsilon;
     int m_precision, m_measureEvery;
+    bool m_batchFlow = true;
+    double m_batchBytes = 32.0 * (1ull << 30);
     std::string m_inputDir = ".", m_outputDir = ".";
     std::vector<std::string> m_inputConfList;
+    std::vector<double> m_obsValues;
 };

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "Math/field.h"
+#include "Observables/observable.h"
 
 namespace LatticeIO {
 struct InputConf {
@@ -19,8 +20,14 @@ struct OutputConf {
     static void writeConf(GluonField &lattice, const char *outputFile);
 };
 struct OutputObs {
-    /// rows of [t, plaq, topc, energy] printed with %.17g (the reference uses %lf,
-    /// src/InputOutput/outputobs.cpp:64-84)
+    /// the reference's header line "Flow Time\tPlaq.\tTopc.\tEnerg.\t", then rows of
+    /// [t, plaq, topc, energy], tab-separated, printed with %.17g where the
+    /// reference uses %lf (src/InputOutput/outputobs.cpp:64-84)
     static void writeFlowObs(const std::string &path, const std::vector<std::array<double, 4>> &rows);
+    /// observables.dat of the ensemble generation (outputobs.cpp:30-55): a
+    /// header "Step\t<name>\t..." and one row per measurement, the step and
+    /// each observable's value() (%.17g, the reference %lf)
+    static void initialize(const std::string &outputDir, std::vector<Observable *> &obsList);
+    static void writeObs(const std::string &outputDir, std::vector<Observable *> &obsList, int MCSteps);
 };
 }  // namespace LatticeIO

@@ -6,7 +6,7 @@
 
 class GpuSuperObs : public Observable {
 public:
-    explicit GpuSuperObs(int precision = LYM_PREC_FP64) : m_precision(precision) { m_name = "superobs"; }
+    explicit GpuSuperObs(int precision = LYM_PREC_FP64) : m_precision(precision) { m_name = "SuperObs"; }  // the reference's name, superobs.cpp:15
     void initObservable(GluonField *field) override { m_field = field; }
     void compute() override;
     /// sum over sites of |q-terms|: the scale of the cancellations in topc

@@ -14,6 +14,16 @@ public:
 
     double plaq = 0, energy = 0, topc = 0;
 
+    /// sets the results from a measurement made elsewhere (a flow series row);
+    /// value() is the plaquette
+    void setResults(double p, double t, double e)
+    {
+        plaq = p;
+        topc = t;
+        energy = e;
+        m_value = p;
+    }
+
 protected:
     GluonField *m_field = nullptr;
     double m_value = 0;

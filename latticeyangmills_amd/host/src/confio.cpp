@@ -65,7 +65,30 @@ void OutputObs::writeFlowObs(const std::string &path, const std::vector<std::arr
 {
     FILE *f = fopen(path.c_str(), "w");
     if (!f) throw std::runtime_error("cannot create " + path);
-    for (auto &r : rows) fprintf(f, "%.17g %.17g %.17g %.17g\n", r[0], r[1], r[2], r[3]);
+    fprintf(f, "Flow Time\tPlaq.\tTopc.\tEnerg.\t\n");
+    for (auto &r : rows) fprintf(f, "%.17g\t%.17g\t%.17g\t%.17g\t\n", r[0], r[1], r[2], r[3]);
+    fclose(f);
+}
+
+static std::string obsFile(const std::string &dir) { return (std::filesystem::path(dir) / "observables.dat").string(); }
+
+void OutputObs::initialize(const std::string &outputDir, std::vector<Observable *> &obsList)
+{
+    FILE *f = fopen(obsFile(outputDir).c_str(), "w");
+    if (!f) throw std::runtime_error("cannot create " + obsFile(outputDir));
+    fprintf(f, "Step\t");
+    for (auto *o : obsList) fprintf(f, "%s\t", o->getName());
+    fprintf(f, "\n");
+    fclose(f);
+}
+
+void OutputObs::writeObs(const std::string &outputDir, std::vector<Observable *> &obsList, int MCSteps)
+{
+    FILE *f = fopen(obsFile(outputDir).c_str(), "a");
+    if (!f) throw std::runtime_error("cannot append to " + obsFile(outputDir));
+    fprintf(f, "%i\t", MCSteps);
+    for (auto *o : obsList) fprintf(f, "%.17g\t", o->value());
+    fprintf(f, "\n");
     fclose(f);
 }
 

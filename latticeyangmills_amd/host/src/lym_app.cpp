@@ -8,7 +8,12 @@
 //   lym_app run INPUT.json     (the reference's own input files: App FLOW or
 //                              PGFG, LatticeSize, Action puregauge, Observables,
 //                              InputDir, OutputDir; optional build keys
-//                              "precision", "measureEvery", "seed", "nHit")
+//                              "precision", "measureEvery", "batchFlow" (FLOW:
+//                              false flows the inputs one by one), "seed",
+//                              "nHit", "candidate" ("exph" | "su2") and
+//                              "referenceMetropolis" (PGFG: true takes the
+//                              reference's 30 hits of R*S*T candidates))
+//   flow also takes --no-batch, generate --candidate exph|su2
 //   lym_app io-selftest DIR   (host only: writes and reads back a .bin file)
 //   lym_app scales FILE VOLUME [BETA]
 //                             (host only: t0/a^2 and w0/a of a flow series file
@@ -16,7 +21,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <random>
 #include <string>
 
 #include "Actions/gpupuregauge.h"
@@ -32,7 +36,8 @@ namespace {
 struct Args {
     std::array<int, 4> lat{{0, 0, 0, 0}};
     double eps = 0.01, tau = 1.0, beta = 6.0;
-    int therm = 0, steps = 0, confs = 1, nhit = 10, every = 1, precision = LYM_PREC_FP64;
+    int therm = 0, steps = 0, confs = 1, nhit = 10, every = 1, precision = LYM_PREC_FP64, candidate = LYM_CAND_EXPH;
+    bool batch = true;
     unsigned long long seed = 0;
     std::string input = ".", output = ".", start = "Cold";
 };
@@ -57,8 +62,13 @@ bool parse(int argc, char **argv, Args &a)
             }
             continue;
         }
+        if (k == "--no-batch") {
+            a.batch = false;
+            continue;
+        }
         if (!(v = next())) return false;
         if (k == "--eps") a.eps = atof(v);
+        else if (k == "--candidate") a.candidate = (std::string(v) == "su2") ? LYM_CAND_SU2 : LYM_CAND_EXPH;
         else if (k == "--tau") a.tau = atof(v);
         else if (k == "--beta") a.beta = atof(v);
         else if (k == "--therm") a.therm = atoi(v);
@@ -105,6 +115,7 @@ int run_json(const char *path)
     if (type == "FLOW") {
         GpuWilsonFlow app(p["tauFinal"].number(), p["epsilon"].number(), precision,
                           j.has("measureEvery") ? j["measureEvery"].integer() : 1);
+        if (j.has("batchFlow")) app.setBatchFlow(j["batchFlow"].b);
         app.setAction(&act);
         if (superobs) app.addObservable(&obs);
         app.createLattice(lat);
@@ -117,9 +128,12 @@ int run_json(const char *path)
     if (type == "PGFG") {
         const int mc = p["MCSteps"].integer();
         const int nconf = p.has("NConf") ? p["NConf"].integer() : mc / p["NCorr"].integer();
+        const bool refMetro = j.has("referenceMetropolis") && j["referenceMetropolis"].b;
+        int nhit = refMetro ? 30 : 10, cand = refMetro ? LYM_CAND_SU2 : LYM_CAND_EXPH;
+        if (j.has("nHit")) nhit = j["nHit"].integer();
+        if (j.has("candidate")) cand = j["candidate"].string() == "su2" ? LYM_CAND_SU2 : LYM_CAND_EXPH;
         GpuGaugeFieldFactory app(mc, p["thermSteps"].integer(), nconf, p["SU3eps"].number(), p["startType"].string(),
-                                 beta, j.has("seed") ? (unsigned long long)j["seed"].number() : 0ull,
-                                 j.has("nHit") ? j["nHit"].integer() : 10);
+                                 beta, j.has("seed") ? (unsigned long long)j["seed"].number() : 0ull, nhit, cand);
         app.setAction(&act);
         if (superobs) app.addObservable(&obs);
         app.createLattice(lat);
@@ -146,7 +160,9 @@ int main(int argc, char **argv)
             FILE *f = fopen(argv[2], "r");
             if (!f) throw std::runtime_error(std::string("cannot read ") + argv[2]);
             std::array<double, 4> r;
-            while (fscanf(f, "%lf %lf %lf %lf", &r[0], &r[1], &r[2], &r[3]) == 4) rows.push_back(r);
+            char line[512];
+            while (fgets(line, sizeof line, f))  // rows of numbers; the header line is skipped
+                if (sscanf(line, "%lf %lf %lf %lf", &r[0], &r[1], &r[2], &r[3]) == 4) rows.push_back(r);
             fclose(f);
             const FlowScales sc = flowScales(rows, atol(argv[3]));
             printf("t0/a^2 = %.12g\nw0/a = %.12g\n", sc.t0, sc.w0);
@@ -160,11 +176,11 @@ int main(int argc, char **argv)
             if (argc < 3) return usage();
             const std::array<int, 4> dims{{8, 8, 8, 20}};  // 10240 sites: more than one I/O block
             GluonField a(dims), b(dims);
-            std::mt19937_64 rng(1);
-            std::uniform_real_distribution<double> u(-1, 1);
+            // a closed form of (mu, site, element), so a reader of the file can
+            // check every value, on both sides of an I/O block boundary
             for (int mu = 0; mu < 4; mu++)
-                for (auto &m : a[mu].lattice)
-                    for (auto &x : m.mat) x = u(rng);
+                for (int s = 0; s < a[mu].sites; s++)
+                    for (int k = 0; k < 18; k++) a[mu].at(s).mat[size_t(k)] = mu + 0.25 * k + 1e-5 * s;
             const std::string f = std::string(argv[2]) + "/selftest.bin";
             LatticeIO::OutputConf::writeConf(a, f.c_str());
             LatticeIO::InputConf::readConf(b, f.c_str());
@@ -181,6 +197,7 @@ int main(int argc, char **argv)
         if (!parse(argc, argv, a) || a.lat[0] < 2) return usage();
         if (cmd == "flow") {
             GpuWilsonFlow app(a.tau, a.eps, a.precision, a.every);
+            app.setBatchFlow(a.batch);
             GpuSuperObs obs(a.precision);
             GpuPureGauge act(a.beta, a.precision);
             app.setAction(&act);
@@ -190,10 +207,11 @@ int main(int argc, char **argv)
             app.setOutputDir(a.output);
             app.execute();
             printf("flowed %zu configuration(s) to t=%g\n", app.inputList().size(), a.tau);
+            printf("last measured row: plaq %.17g topc %.17g energy %.17g\n", obs.plaq, obs.topc, obs.energy);
             return 0;
         }
         if (cmd == "generate") {
-            GpuGaugeFieldFactory app(a.steps, a.therm, a.confs, a.eps, a.start, a.beta, a.seed, a.nhit);
+            GpuGaugeFieldFactory app(a.steps, a.therm, a.confs, a.eps, a.start, a.beta, a.seed, a.nhit, a.candidate);
             GpuSuperObs obs(LYM_PREC_FP64);
             GpuPureGauge act(a.beta);
             app.setAction(&act);

@@ -1,13 +1,10 @@
-// Host SU(3) algebra and the DeviceField registry.
+// Host SU(3) algebra (no device code: the lattice-wide hot path runs in the
+// HIP kernels behind include/lym.h).
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
-#include <map>
-#include <memory>
-#include <mutex>
 
-#include "Math/devicefield.h"
 #include "Math/lattice.h"
 #include "Math/su3.h"
 
@@ -126,76 +123,4 @@ void setToZero(Lattice<SU3> &l)
 void setToIdentity(Lattice<SU3> &l)
 {
     for (auto &m : l.lattice) m.setSU3Identity();
-}
-
-// ------------------------------------------------------------ DeviceField ---
-
-namespace {
-std::map<GluonField *, std::unique_ptr<DeviceField>> &registry()
-{
-    static std::map<GluonField *, std::unique_ptr<DeviceField>> r;
-    return r;
-}
-// Guards the registry only: a DeviceField itself, like its lym_ctx, is used
-// from one thread at a time.
-std::mutex &registry_lock()
-{
-    static std::mutex m;
-    return m;
-}
-}  // namespace
-
-DeviceField &DeviceField::of(GluonField *field, int precision, int device)
-{
-    if (!field) throw std::runtime_error("DeviceField: null GluonField");
-    std::lock_guard<std::mutex> g(registry_lock());
-    auto &r = registry();
-    auto it = r.find(field);
-    if (it == r.end()) it = r.emplace(field, std::unique_ptr<DeviceField>(new DeviceField(field, precision, device))).first;
-    return *it->second;
-}
-
-void DeviceField::release(GluonField *field)
-{
-    std::lock_guard<std::mutex> g(registry_lock());
-    registry().erase(field);
-}
-
-DeviceField::DeviceField(GluonField *field, int precision, int device) : m_host(field), m_precision(precision)
-{
-    int rc = lym_create(&m_ctx, field->m_size.data(), precision, device);
-    if (rc != LYM_OK) throw std::runtime_error("lym_create failed with code " + std::to_string(rc));
-}
-
-DeviceField::~DeviceField() { lym_destroy(m_ctx); }
-
-void DeviceField::check(int rc, const char *what)
-{
-    if (rc != LYM_OK) throw std::runtime_error(std::string(what) + ": " + lym_last_error(m_ctx));
-}
-
-void DeviceField::toDevice()
-{
-    if (!m_hostNewer) return;
-    const double *p[4];
-    for (int mu = 0; mu < 4; mu++) p[mu] = (*m_host)[mu].lattice.data()->mat.data();
-    check(lym_upload_links(m_ctx, p), "lym_upload_links");
-    m_hostNewer = false;
-}
-
-void DeviceField::toHost()
-{
-    if (!m_deviceNewer) return;
-    double *p[4];
-    for (int mu = 0; mu < 4; mu++) p[mu] = (*m_host)[mu].lattice.data()->mat.data();
-    check(lym_download_links(m_ctx, p), "lym_download_links");
-    m_deviceNewer = false;
-}
-
-DeviceField::Health DeviceField::health()
-{
-    toDevice();
-    Health h{0.0, 0};
-    check(lym_check_links(m_ctx, &h.maxDeviation, &h.nonFinite), "lym_check_links");
-    return h;
 }

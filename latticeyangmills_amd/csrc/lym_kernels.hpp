@@ -241,26 +241,25 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     Nbr n = neighbours(g, s);
     su3<S> om = staple_from<S, MU>(src, s, n);
     su3<S> U = src.template get<S, 0, 0, 0>(MU, s);
-    su3<S> X = mul_nn(U, om);
-    herm8<double> z = force_from_x(X);
+    herm8<double> z = force_from_product(U, om);  // Z of X = U Omega
     S *am = acc + (long)MU * 8 * V;
     herm8<double> q;
     if constexpr (STAGE == 1) {
-        store_h8<S>(am, V, unsigned(s), z);
+        store_h8<S, kStreamAux>(am, V, unsigned(s), z);
         q = h8_scale(eps * 0.25, z);
     } else if constexpr (STAGE == 2) {
-        herm8<double> a = load_h8<S>(am, V, unsigned(s));
+        herm8<double> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
         a = h8_axpby(8.0 / 9.0, z, -17.0 / 36.0, a);
-        store_h8<S>(am, V, unsigned(s), a);
+        store_h8<S, kStreamAux>(am, V, unsigned(s), a);
         q = h8_scale(eps, a);
     } else {
-        herm8<double> a = load_h8<S>(am, V, unsigned(s));
+        herm8<double> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
         q = h8_scale(eps, h8_axpby(0.75, z, -1.0, a));
     }
-    su3<double> E = exp_iq(q);
+    su3<double> E = (sizeof(S) == 4) ? exp_iq<kShortSeriesTermsF32, kSeriesTermsF32>(q) : exp_iq(q);
     su3<double> W = mul_nn(E, convert<double>(U));
     if constexpr (REUNIT) reunitarize(W);
-    store_link<S, double>(wout + (long)MU * 9 * V, V, unsigned(s), W);
+    store_link<S, double, kStreamAux>(wout + (long)MU * 9 * V, V, unsigned(s), W);
 }
 
 // One 64-site run, all four directions (wave = direction).  LDS selects what
@@ -797,7 +796,7 @@ LYM_DI void export_force_site(const F2<S> *__restrict__ f, const Geom g, long s,
         return;
     }
     su3<double> U = load_link<double, S>(f + (long)MU * 9 * g.V, g.V, s);
-    herm8<double> z = force_from_x(mul_nn(U, om));
+    herm8<double> z = force_from_product(U, om);
     const double d2 = -z.d0 - z.d1;
     o[0] = z.d0;  o[1] = 0.0;
     o[8] = z.d1;  o[9] = 0.0;

@@ -7,7 +7,8 @@
 // host threads copy chunk k-1 out of the other into the caller's array, so
 // the page faults are taken by several threads at once; uploads run the same
 // pipeline the other way.  The buffers are allocated with hipHostMalloc on
-// first use and kept until release().
+// first use, sized to the transfer (at most kBounceBytes each), and kept
+// until release().  Nothing here throws: these run inside the C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -18,40 +19,57 @@
 
 namespace lym {
 
-constexpr size_t kBounceBytes = size_t(64) << 20;  // per buffer
+constexpr size_t kBounceBytes = size_t(64) << 20;  // per buffer, at most
 constexpr unsigned kCopyThreads = 8;
 
 struct Bounce {
     char *buf[2] = {nullptr, nullptr};
+    size_t cap = 0;  // bytes per buffer
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool pending[2] = {false, false};  // an upload's DMA may still read buf[k] (ev[k] marks its end)
 
-    hipError_t init()
+    // Buffers of at least min(kBounceBytes, need) bytes: a small lattice pins
+    // little; a larger transfer later replaces them, after their DMAs end.
+    hipError_t init(size_t need)
     {
-        for (int k = 0; k < 2; k++) {
-            if (!buf[k]) {
-                void *p = nullptr;
-                if (hipError_t e = hipHostMalloc(&p, kBounceBytes, hipHostMallocDefault)) return e;
-                buf[k] = static_cast<char *>(p);
-            }
+        const size_t want = std::min(kBounceBytes, std::max<size_t>(need, 1));
+        for (int k = 0; k < 2; k++)
             if (!ev[k])
                 if (hipError_t e = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming)) return e;
+        if (cap >= want) return hipSuccess;
+        for (int k = 0; k < 2; k++) {
+            if (pending[k]) {
+                if (hipError_t e = hipEventSynchronize(ev[k])) return e;
+                pending[k] = false;
+            }
+            if (buf[k]) (void)hipHostFree(buf[k]);
+            buf[k] = nullptr;
         }
+        cap = 0;
+        for (int k = 0; k < 2; k++) {
+            void *p = nullptr;
+            if (hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault)) return e;
+            buf[k] = static_cast<char *>(p);
+        }
+        cap = want;
         return hipSuccess;
     }
     void release()
     {
         for (int k = 0; k < 2; k++) {
+            if (pending[k] && ev[k]) (void)hipEventSynchronize(ev[k]);
             if (buf[k]) (void)hipHostFree(buf[k]);
             if (ev[k]) (void)hipEventDestroy(ev[k]);
             buf[k] = nullptr;
             ev[k] = nullptr;
             pending[k] = false;
         }
+        cap = 0;
     }
 };
 
-// memcpy split over up to kCopyThreads threads (one for small copies)
+// memcpy split over up to kCopyThreads threads (one for small copies).  If a
+// thread cannot be created, the calling thread copies the parts no thread took.
 inline void parallel_memcpy(char *dst, const char *src, size_t n)
 {
     const unsigned nt = std::min(kCopyThreads, std::max(1u, std::thread::hardware_concurrency()));
@@ -60,10 +78,16 @@ inline void parallel_memcpy(char *dst, const char *src, size_t n)
         return;
     }
     std::vector<std::thread> th;
-    th.reserve(nt);
-    for (unsigned i = 0; i < nt; i++) {
-        const size_t a = n * i / nt, b = n * (i + 1) / nt;
-        th.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+    unsigned started = 0;
+    try {
+        th.reserve(nt);
+        for (; started < nt; started++) {
+            const size_t a = n * started / nt, b = n * (started + 1) / nt;
+            th.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+        }
+    } catch (...) {
+        const size_t a = n * started / nt;
+        std::memcpy(dst + a, src + a, n - a);
     }
     for (auto &t : th) t.join();
 }
@@ -74,18 +98,18 @@ inline void parallel_memcpy(char *dst, const char *src, size_t n)
 // copied out in iteration k - 1) and copies chunk k - 1 out of the other.
 inline hipError_t d2h_bounced(Bounce &bb, hipStream_t s, void *dst, const void *src, size_t n)
 {
-    if (hipError_t e = bb.init()) return e;
-    const size_t nch = (n + kBounceBytes - 1) / kBounceBytes;
+    if (hipError_t e = bb.init(n)) return e;
+    const size_t chunk = bb.cap, nch = (n + chunk - 1) / chunk;
     for (size_t k = 0; k <= nch; k++) {
         if (k < nch) {
-            const size_t off = k * kBounceBytes, len = std::min(kBounceBytes, n - off);
+            const size_t off = k * chunk, len = std::min(chunk, n - off);
             if (hipError_t e = hipMemcpyAsync(bb.buf[k & 1], static_cast<const char *>(src) + off, len,
                                               hipMemcpyDeviceToHost, s))
                 return e;
             if (hipError_t e = hipEventRecord(bb.ev[k & 1], s)) return e;
         }
         if (k > 0) {
-            const size_t j = k - 1, off = j * kBounceBytes, len = std::min(kBounceBytes, n - off);
+            const size_t j = k - 1, off = j * chunk, len = std::min(chunk, n - off);
             if (hipError_t e = hipEventSynchronize(bb.ev[j & 1])) return e;
             bb.pending[j & 1] = false;
             parallel_memcpy(static_cast<char *>(dst) + off, bb.buf[j & 1], len);
@@ -101,11 +125,11 @@ inline hipError_t d2h_bounced(Bounce &bb, hipStream_t s, void *dst, const void *
 // last DMAs pending (the next use of a buffer waits for them).
 inline hipError_t h2d_bounced(Bounce &bb, hipStream_t s, void *dst, const void *src, size_t n)
 {
-    if (hipError_t e = bb.init()) return e;
-    const size_t nch = (n + kBounceBytes - 1) / kBounceBytes;
+    if (hipError_t e = bb.init(n)) return e;
+    const size_t chunk = bb.cap, nch = (n + chunk - 1) / chunk;
     for (size_t k = 0; k < nch; k++) {
         const int b = int(k & 1);
-        const size_t off = k * kBounceBytes, len = std::min(kBounceBytes, n - off);
+        const size_t off = k * chunk, len = std::min(chunk, n - off);
         if (bb.pending[b]) {
             if (hipError_t e = hipEventSynchronize(bb.ev[b])) return e;
             bb.pending[b] = false;

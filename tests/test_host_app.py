@@ -54,8 +54,9 @@ def test_generate_then_flow(tmp_path):
     confs, out = tmp_path / "confs", tmp_path / "out"
     run("generate", "--lattice", *dims, "--beta", 6.0, "--eps", 0.24, "--therm", 5, "--steps", 4, "--confs", 2,
         "--start", "Hot", "--seed", 3, "--output", confs)
-    files = sorted(os.listdir(confs))
+    files = sorted(f for f in os.listdir(confs) if f.endswith(".bin"))
     assert files == ["conf0000.bin", "conf0001.bin"]
+    assert os.path.isfile(confs / "observables.dat")
     # conf0000 = hot start + 5 thermalisation + 2 sampling sweeps (NCorr = steps/confs)
     with LatticeContext(dims, "fp64") as ctx:
         ctx.set_hot(3)

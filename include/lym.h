@@ -31,6 +31,7 @@ extern "C" {
 #define LYM_ERR_HIP (-2)    /* HIP runtime error */
 #define LYM_ERR_NOMEM (-3)  /* device allocation failed */
 #define LYM_ERR_NODEV (-4)  /* no GPU visible */
+#define LYM_ERR_TOO_LARGE (-5) /* lattice beyond a fixed maximum (lym_observables_noalloc) */
 
 #define LYM_PREC_FP64 0  /* FP64 links, FP64 arithmetic throughout */
 #define LYM_PREC_MIXED 1 /* FP32 links/staples/clover; FP64 exp, reunitarisation, reductions */
@@ -46,6 +47,8 @@ typedef struct lym_ctx lym_ctx;
  * App::createLattice (src/Apps/wilsonflow.cpp:87-91, src/Apps/app.cpp:41-44). */
 int lym_create(lym_ctx **ctx, const int dims[4], int precision, int device);
 int lym_destroy(lym_ctx *ctx);
+/* The message of the last failure on ctx; with ctx == NULL, that of the last
+ * failed lym_observables_noalloc call on this thread ("null context" if none). */
 const char *lym_last_error(const lym_ctx *ctx);
 /* V = number of sites; precision as given to lym_create. */
 int lym_info(const lym_ctx *ctx, int dims[4], int64_t *volume, int *precision);
@@ -150,6 +153,31 @@ int lym_set_schedule(lym_ctx *ctx, int kind, int target);
  * NULL) holds one line per lattice shape and precision; summary receives the
  * choice. */
 int lym_autotune(lym_ctx *ctx, const char *cache_path, char *summary, int summary_len);
+
+/* No counterpart in the reference: SuperObs::compute (src/Observables/superobs.cpp:31-128)
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  It is
+ * lym_observables on a link field the caller already holds on the device,
+ * and it calls no hipMalloc, hipFree or allocating library routine: its one
+ * scratch buffer, the per-workgroup partial sums, is a file-scope device
+ * array of LYM_NOALLOC_MAX_SITES / 64 slots (16 MiB) that every GPU holds
+ * until the process ends.
+ *   links:  the field on the calling thread's current device, in the device
+ *           layout of the library: direction planes mu = 0..3, each 9
+ *           complex elements e = 3i + j by V sites,
+ *           links[((mu*9 + e)*V + s)] = (Re U_ij, Im U_ij) of site s, as
+ *           double2 (LYM_PREC_FP64) or float2 (LYM_PREC_MIXED);
+ *   out:    4 doubles in device memory, written as lym_observables' out;
+ *   stream: the hipStream_t to run on (NULL: the null stream); the call is
+ *           asynchronous on it.
+ * Extents >= 2 and V <= LYM_NOALLOC_MAX_SITES, else LYM_ERR_ARG or
+ * LYM_ERR_TOO_LARGE before any device work; the message of a failure is
+ * lym_last_error(NULL) on the calling thread.  Every call on one GPU shares
+ * that GPU's scratch array (each call writes every slot it reads, so nothing
+ * needs clearing between calls), so calls that overlap on one GPU, on
+ * different streams or threads, race. */
+#define LYM_NOALLOC_MAX_SITES (1 << 24)
+int lym_observables_noalloc(const void *links, const int dims[4], int precision, double *out, void *stream);
 
 /* Measurement helper with no counterpart in the reference: a streaming
  * device-to-device copy of `bytes` bytes repeated `reps` times on GPU

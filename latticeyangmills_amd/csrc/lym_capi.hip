@@ -18,6 +18,10 @@
 using lym::Geom;
 using lym::ksum;
 
+namespace lym {
+const char *noalloc_last_error();  // lym_noalloc.hip
+}
+
 // Metropolis blocks in column order (lym::metro_column_schedule): each XCD
 // marches (z, y) columns of about this many sites of one parity along t, so
 // the neighbouring time slices of a column stay in its 4 MB L2.  A time
@@ -333,7 +337,12 @@ int lym_destroy(lym_ctx *c)
     return LYM_OK;
 }
 
-const char *lym_last_error(const lym_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *lym_last_error(const lym_ctx *c)
+{
+    if (c) return c->err.c_str();
+    const char *m = lym::noalloc_last_error();  // context-free calls (lym_observables_noalloc)
+    return m ? m : "null context";
+}
 
 int lym_info(const lym_ctx *c, int dims[4], int64_t *volume, int *precision)
 {

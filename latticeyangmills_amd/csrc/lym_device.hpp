@@ -611,16 +611,19 @@ template <int N_SHORT = kShortSeriesTerms, int N_LONG = kSeriesTerms> LYM_DI su3
 
 // Z = (i/2) [X - X^+]_traceless for X = U * Omega
 // (src/Actions/puregauge.cpp:62-79, there formed as M = Omega^+ U^+, D = M^+ - M).
-template <typename R> LYM_DI herm8<double> force_from_x(const su3<R> &X)
+// In the precision T of the result: FP64 on the FP64 path; the mixed path
+// keeps Z and the RK accumulator in FP32 (its storage precision) and moves to
+// FP64 at the exponential.
+template <typename R, typename T = double> LYM_DI herm8<T> force_from_x(const su3<R> &X)
 {
-    double i0 = double(X.e[0].im), i1 = double(X.e[4].im), i2 = double(X.e[8].im);
-    double tr = (i0 + i1 + i2) * (1.0 / 3.0);
-    herm8<double> z;
+    T i0 = T(X.e[0].im), i1 = T(X.e[4].im), i2 = T(X.e[8].im);
+    T tr = (i0 + i1 + i2) * T(1.0 / 3.0);
+    herm8<T> z;
     z.d0 = -(i0 - tr);
     z.d1 = -(i1 - tr);
     // Z_ij = (i/2)(X_ij - conj(X_ji)) for i < j
     auto off = [](cplx<R> a, cplx<R> b) {
-        return cplx<double>{-0.5 * (double(a.im) + double(b.im)), 0.5 * (double(a.re) - double(b.re))};
+        return cplx<T>{T(-0.5) * (T(a.im) + T(b.im)), T(0.5) * (T(a.re) - T(b.re))};
     };
     z.o01 = off(X.e[1], X.e[3]);
     z.o02 = off(X.e[2], X.e[6]);
@@ -653,7 +656,7 @@ template <typename R> LYM_DI R mul_entry_im(const su3<R> &a, const su3<R> &b, in
 // Z = force_from_x(U * Omega) from the entries of X = U * Omega it reads: the
 // imaginary parts of the diagonal and the off-diagonal pairs (X_ij, X_ji),
 // 90 instead of 216 arithmetic operations, the same values.
-template <typename R> LYM_DI herm8<double> force_from_product(const su3<R> &U, const su3<R> &om)
+template <typename R, typename T = double> LYM_DI herm8<T> force_from_product(const su3<R> &U, const su3<R> &om)
 {
     su3<R> X;
 #pragma unroll
@@ -664,54 +667,55 @@ template <typename R> LYM_DI herm8<double> force_from_product(const su3<R> &U, c
     X.e[6] = mul_entry(U, om, 2, 0);
     X.e[5] = mul_entry(U, om, 1, 2);
     X.e[7] = mul_entry(U, om, 2, 1);
-    return force_from_x(X);
+    return force_from_x<R, T>(X);
 }
 
-LYM_DI herm8<double> h8_axpby(double a, const herm8<double> &x, double b, const herm8<double> &y)
+template <typename R> LYM_DI herm8<R> h8_axpby(R a, const herm8<R> &x, R b, const herm8<R> &y)
 {
-    herm8<double> r;
+    herm8<R> r;
     r.d0 = a * x.d0 + b * y.d0;
     r.d1 = a * x.d1 + b * y.d1;
-    r.o01 = cplx<double>{a * x.o01.re + b * y.o01.re, a * x.o01.im + b * y.o01.im};
-    r.o02 = cplx<double>{a * x.o02.re + b * y.o02.re, a * x.o02.im + b * y.o02.im};
-    r.o12 = cplx<double>{a * x.o12.re + b * y.o12.re, a * x.o12.im + b * y.o12.im};
+    r.o01 = cplx<R>{a * x.o01.re + b * y.o01.re, a * x.o01.im + b * y.o01.im};
+    r.o02 = cplx<R>{a * x.o02.re + b * y.o02.re, a * x.o02.im + b * y.o02.im};
+    r.o12 = cplx<R>{a * x.o12.re + b * y.o12.re, a * x.o12.im + b * y.o12.im};
     return r;
 }
-LYM_DI herm8<double> h8_scale(double a, const herm8<double> &x)
+// a * x in FP64 (the generator handed to the exponential)
+template <typename R> LYM_DI herm8<double> h8_scale(double a, const herm8<R> &x)
 {
     herm8<double> r;
-    r.d0 = a * x.d0;
-    r.d1 = a * x.d1;
-    r.o01 = cplx<double>{a * x.o01.re, a * x.o01.im};
-    r.o02 = cplx<double>{a * x.o02.re, a * x.o02.im};
-    r.o12 = cplx<double>{a * x.o12.re, a * x.o12.im};
+    r.d0 = a * double(x.d0);
+    r.d1 = a * double(x.d1);
+    r.o01 = cplx<double>{a * double(x.o01.re), a * double(x.o01.im)};
+    r.o02 = cplx<double>{a * double(x.o02.re), a * double(x.o02.im)};
+    r.o12 = cplx<double>{a * double(x.o12.re), a * double(x.o12.im)};
     return r;
 }
 
 // RK accumulator in HBM: acc[(mu*8 + k)*V + s], k = d0, d1, o01.re, o01.im,
 // o02.re, o02.im, o12.re, o12.im (buffer addressing as load_link)
-template <typename S, int AUX = 0> LYM_DI herm8<double> load_h8(const S *__restrict__ a, long V, unsigned s)
+template <typename S, int AUX = 0> LYM_DI herm8<S> load_h8(const S *__restrict__ a, long V, unsigned s)
 {
     const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
     const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
-    double v[8];
+    S v[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = double(buf_load<S, AUX>(r, off, unsigned(k) * es));
-    herm8<double> z;
+    for (int k = 0; k < 8; k++) v[k] = buf_load<S, AUX>(r, off, unsigned(k) * es);
+    herm8<S> z;
     z.d0 = v[0];
     z.d1 = v[1];
-    z.o01 = cplx<double>{v[2], v[3]};
-    z.o02 = cplx<double>{v[4], v[5]};
-    z.o12 = cplx<double>{v[6], v[7]};
+    z.o01 = cplx<S>{v[2], v[3]};
+    z.o02 = cplx<S>{v[4], v[5]};
+    z.o12 = cplx<S>{v[6], v[7]};
     return z;
 }
-template <typename S, int AUX = 0> LYM_DI void store_h8(S *__restrict__ a, long V, unsigned s, const herm8<double> &z)
+template <typename S, int AUX = 0> LYM_DI void store_h8(S *__restrict__ a, long V, unsigned s, const herm8<S> &z)
 {
     const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
     const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
-    const double v[8] = {z.d0, z.d1, z.o01.re, z.o01.im, z.o02.re, z.o02.im, z.o12.re, z.o12.im};
+    const S v[8] = {z.d0, z.d1, z.o01.re, z.o01.im, z.o02.re, z.o02.im, z.o12.re, z.o12.im};
 #pragma unroll
-    for (int k = 0; k < 8; k++) buf_store<S, AUX>(S(v[k]), r, off, unsigned(k) * es);
+    for (int k = 0; k < 8; k++) buf_store<S, AUX>(v[k], r, off, unsigned(k) * es);
 }
 
 // Project a near-unitary matrix back onto SU(3): Gram-Schmidt on rows 0, 1,

@@ -241,20 +241,21 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     Nbr n = neighbours(g, s);
     su3<S> om = staple_from<S, MU>(src, s, n);
     su3<S> U = src.template get<S, 0, 0, 0>(MU, s);
-    herm8<double> z = force_from_product(U, om);  // Z of X = U Omega
+    // Z of X = U Omega and the RK combination in the storage precision S
+    const herm8<S> z = force_from_product<S, S>(U, om);
     S *am = acc + (long)MU * 8 * V;
     herm8<double> q;
     if constexpr (STAGE == 1) {
         store_h8<S, kStreamAux>(am, V, unsigned(s), z);
         q = h8_scale(eps * 0.25, z);
     } else if constexpr (STAGE == 2) {
-        herm8<double> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
-        a = h8_axpby(8.0 / 9.0, z, -17.0 / 36.0, a);
+        herm8<S> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
+        a = h8_axpby(S(8.0 / 9.0), z, S(-17.0 / 36.0), a);
         store_h8<S, kStreamAux>(am, V, unsigned(s), a);
         q = h8_scale(eps, a);
     } else {
-        herm8<double> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
-        q = h8_scale(eps, h8_axpby(0.75, z, -1.0, a));
+        const herm8<S> a = load_h8<S, kStreamAux>(am, V, unsigned(s));
+        q = h8_scale(eps, h8_axpby(S(0.75), z, S(-1.0), a));
     }
     su3<double> E = (sizeof(S) == 4) ? exp_iq<kShortSeriesTermsF32, kSeriesTermsF32>(q) : exp_iq(q);
     su3<double> W = mul_nn(E, convert<double>(U));

@@ -154,6 +154,8 @@ inline std::vector<std::pair<int, int>> schedule_candidates()
     for (int lds : {0, SCHED_LDS})
         for (auto k : std::vector<std::pair<int, int>>{{SCHED_LINEAR, 0},
                                                        {SCHED_CONTIG, 0},
+                                                       {SCHED_COLUMN, 256},
+                                                       {SCHED_COLUMN, 512},
                                                        {SCHED_COLUMN, 1024},
                                                        {SCHED_COLUMN, 2048},
                                                        {SCHED_COLUMN, 4096},

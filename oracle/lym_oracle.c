@@ -325,6 +325,145 @@ int or_flow_step(double *links, const int dims[4], double eps)
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* The same flow step in the reference's whole-lattice structure: every      */
+/* operation of computeStaples / computeDerivative / flowStep               */
+/* (src/Actions/puregauge.cpp:36-79, src/Apps/wilsonflow.cpp:141-172) is a   */
+/* pass over a whole-lattice temporary, the MPI halo shift                   */
+/* (include/Math/lattice.h:360-594) a periodic copy.  Per site it does the   */
+/* arithmetic of or_flow_step in the same order, so the results are equal    */
+/* bit for bit; it exists to time the reference's memory structure as the    */
+/* CPU baseline.                                                             */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    geom g;
+    long n; /* V * 18 */
+    double *shiftmu, *tmp, *tmp2, *omega, *m, *ma;
+} lat_work;
+
+/* out(x) = in(x + step e_dir) */
+static void lat_shift(const double *in, const geom *g, int dir, int step, double *out)
+{
+    for (long s = 0; s < g->V; s++) memcpy(out + s * 18, in + nbr(g, s, dir, step) * 18, 18 * sizeof(double));
+}
+static void lat_adj(const double *in, long V, double *out)
+{
+    for (long s = 0; s < V; s++) or_su3_adj(in + s * 18, out + s * 18);
+}
+/* a(x) = a(x) b(x) */
+static void lat_mul_into(double *a, const double *b, long V)
+{
+    double r[18];
+    for (long s = 0; s < V; s++) {
+        or_su3_mul(a + s * 18, b + s * 18, r);
+        memcpy(a + s * 18, r, sizeof r);
+    }
+}
+static void lat_add_into(double *a, const double *b, long n)
+{
+    for (long k = 0; k < n; k++) a[k] += b[k];
+}
+static void lat_scale(const double *in, double c, long n, double *out)
+{
+    for (long k = 0; k < n; k++) out[k] = in[k] * c;
+}
+
+/* computeStaples(mu) into w->omega (puregauge.cpp:36-54) */
+static void lat_staples(const double *links, lat_work *w, int mu)
+{
+    const geom *g = &w->g;
+    const double *U_mu = links + (long)mu * w->n;
+    memset(w->omega, 0, sizeof(double) * w->n);
+    for (int nu = 0; nu < 4; nu++) {
+        if (nu == mu) continue;
+        const double *U_nu = links + (long)nu * w->n;
+        lat_shift(U_nu, g, mu, 1, w->shiftmu);          /* shiftmu = shift(U_nu, mu, +1) */
+        memcpy(w->tmp, w->shiftmu, sizeof(double) * w->n);
+        lat_shift(U_mu, g, nu, 1, w->tmp2);              /* tempStaple *= adj(shift(U_mu, nu, +1)) */
+        lat_adj(w->tmp2, g->V, w->m);
+        lat_mul_into(w->tmp, w->m, g->V);
+        lat_adj(U_nu, g->V, w->m);                       /* tempStaple *= adj(U_nu) */
+        lat_mul_into(w->tmp, w->m, g->V);
+        lat_add_into(w->omega, w->tmp, w->n);
+        lat_adj(w->shiftmu, g->V, w->tmp);               /* tempStaple = adj(shiftmu) */
+        lat_adj(U_mu, g->V, w->m);                       /* tempStaple *= adj(U_mu) */
+        lat_mul_into(w->tmp, w->m, g->V);
+        memcpy(w->m, U_nu, sizeof(double) * w->n);       /* tempStaple *= U_nu */
+        lat_mul_into(w->tmp, w->m, g->V);
+        lat_shift(w->tmp, g, nu, -1, w->tmp2);           /* omega += shift(tempStaple, nu, -1) */
+        lat_add_into(w->omega, w->tmp2, w->n);
+    }
+}
+
+/* computeDerivative(mu) into z (puregauge.cpp:62-79) */
+static void lat_derivative(const double *links, lat_work *w, int mu, double *z)
+{
+    const long V = w->g.V;
+    lat_staples(links, w, mu);
+    lat_adj(w->omega, V, w->m);                                /* m = adj(omega) * adj(U_mu) */
+    lat_adj(links + (long)mu * w->n, V, w->tmp);
+    lat_mul_into(w->m, w->tmp, V);
+    lat_adj(w->m, V, w->ma);                                   /* m = adj(m) - m */
+    for (long k = 0; k < w->n; k++) w->ma[k] -= w->m[k];
+    for (long s = 0; s < V; s++) {
+        double *d = w->ma + s * 18, *zs = z + s * 18;
+        double tr = imtr3(d) / 3.0;
+        for (int k = 1; k < 18; k += 8) d[k] -= tr;
+        for (int k = 0; k < 18; k += 2) {
+            double tmp = d[k];
+            zs[k] = -d[k + 1] * 0.5;
+            zs[k + 1] = tmp * 0.5;
+        }
+    }
+}
+
+/* U_mu = exp(Z_mu * c1 * c2) * U_mu over the whole lattice */
+static void lat_exp_update(double *links, lat_work *w, int mu, const double *z, double c1, double c2)
+{
+    const long V = w->g.V;
+    lat_scale(z, c1, w->n, w->tmp);
+    lat_scale(w->tmp, c2, w->n, w->tmp2);
+    for (long s = 0; s < V; s++) or_su3_exp(w->tmp2 + s * 18, w->m + s * 18);
+    double *u = links + (long)mu * w->n;
+    lat_mul_into(w->m, u, V);
+    memcpy(u, w->m, sizeof(double) * w->n);
+}
+
+int or_flow_step_lattice(double *links, const int dims[4], double eps)
+{
+    lat_work w;
+    w.g = make_geom(dims);
+    w.n = w.g.V * 18;
+    double *Z = (double *)malloc(sizeof(double) * 4 * w.n), *D = (double *)malloc(sizeof(double) * w.n);
+    double **bufs[6] = {&w.shiftmu, &w.tmp, &w.tmp2, &w.omega, &w.m, &w.ma};
+    int ok = Z && D;
+    for (int i = 0; i < 6; i++) {
+        *bufs[i] = ok ? (double *)malloc(sizeof(double) * w.n) : NULL;
+        ok = ok && *bufs[i];
+    }
+    if (ok) {
+        for (int mu = 0; mu < 4; mu++) lat_derivative(links, &w, mu, Z + mu * w.n);
+        for (int mu = 0; mu < 4; mu++) lat_exp_update(links, &w, mu, Z + mu * w.n, eps, 1.0 / 4.0);
+        for (int mu = 0; mu < 4; mu++) {
+            double *z = Z + mu * w.n;
+            lat_derivative(links, &w, mu, D);
+            for (long k = 0; k < w.n; k++) z[k] = D[k] * (8.0 / 9.0) - z[k] * (17.0 / 36.0);
+        }
+        for (int mu = 0; mu < 4; mu++) lat_exp_update(links, &w, mu, Z + mu * w.n, eps, 1.0);
+        for (int mu = 0; mu < 4; mu++) {
+            double *z = Z + mu * w.n;
+            lat_derivative(links, &w, mu, D);
+            for (long k = 0; k < w.n; k++) z[k] = D[k] * (3.0 / 4.0) - z[k];
+        }
+        for (int mu = 0; mu < 4; mu++) lat_exp_update(links, &w, mu, Z + mu * w.n, eps, 1.0);
+    }
+    for (int i = 0; i < 6; i++) free(*bufs[i]);
+    free(Z);
+    free(D);
+    return ok ? 0 : -1;
+}
+
 /* Number of series rows, int(tau/eps) (src/Apps/wilsonflow.cpp:117);
  * the flow runs rows-1 steps (wilsonflow.cpp:124). */
 int or_flow_rows(double tau, double eps) { return (int)(tau / eps); }

@@ -34,6 +34,9 @@ void or_derivative(const double *links, const int dims[4], int mu, double *z);
 /* ---- flow (src/Apps/wilsonflow.cpp) ---- */
 int  or_flow_step(double *links, const int dims[4], double eps);
 int  or_flow_rows(double tau, double eps);
+/* or_flow_step in the reference's whole-lattice-temporary structure (equal
+ * bit for bit; the CPU baseline) */
+int  or_flow_step_lattice(double *links, const int dims[4], double eps);
 int  or_flow(double *links, const int dims[4], double eps, double tau,
              int measure_every, int sum_mode, double *series);
 

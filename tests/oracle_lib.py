@@ -34,6 +34,8 @@ def lib():
         L.or_flow.restype = ctypes.c_int
         L.or_flow.argtypes = [_D, _I4, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _D]
         L.or_flow_step.argtypes = [_D, _I4, ctypes.c_double]
+        L.or_flow_step_lattice.restype = ctypes.c_int
+        L.or_flow_step_lattice.argtypes = [_D, _I4, ctypes.c_double]
         L.or_observables.argtypes = [_D, _I4, ctypes.c_int, _D]
         L.or_staples.argtypes = [_D, _I4, ctypes.c_int, _D]
         L.or_derivative.argtypes = [_D, _I4, ctypes.c_int, _D]
@@ -121,6 +123,13 @@ def derivative(links, dims, mu):
 def flow_step(links, dims, eps):
     links = np.array(links, copy=True)
     assert lib().or_flow_step(_p(links), _I4(*dims), eps) == 0
+    return links
+
+
+def flow_step_lattice(links, dims, eps):
+    """or_flow_step in the reference's whole-lattice-temporary structure"""
+    links = np.array(links, copy=True)
+    assert lib().or_flow_step_lattice(_p(links), _I4(*dims), eps) == 0
     return links
 
 

@@ -205,3 +205,13 @@ def test_flow_time_is_heat_equation_time():
     kh2 = 4 * np.sin(k / 2) ** 2
     for t, _, _, e in series[::25]:
         assert abs(e / series[0, 3] - np.exp(-2 * kh2 * t)) < 1e-5, t
+
+
+@pytest.mark.parametrize("dims,eps", [((2, 4, 4, 4), 0.02), ((4, 4, 4, 6), 0.3), ((3, 4, 5, 2), 0.05)])
+def test_whole_lattice_structure_equals_per_site(dims, eps):
+    """The oracle's restatement in the reference's whole-lattice structure
+    (shift copies and lattice temporaries, puregauge.cpp:36-79,
+    wilsonflow.cpp:141-172), the CPU baseline's code, gives the per-site
+    oracle's RK3 step bit for bit, odd extents and periodic wraps included."""
+    links = orc.thermalised(dims, beta=6.0, sweeps=2, seed=3) if all(d % 2 == 0 for d in dims) else orc.hot_start(dims, 3)
+    np.testing.assert_array_equal(orc.flow_step_lattice(links, dims, eps), orc.flow_step(links, dims, eps))

@@ -139,14 +139,21 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
  * (workgroup b takes 64-site run b), 1 = LYM_SCHED_CONTIG (each of the 8 XCDs
  * takes a contiguous eighth), 2 = LYM_SCHED_COLUMN (each XCD marches columns of
  * about `target` sites of the (x, y, z) volume along t, so neighbouring time
- * slices stay in its L2).  Results agree across schedules to rounding: the
- * exponential's series length is chosen per wavefront, and a schedule decides
- * which sites share one.  Default: COLUMN | LDS with 8192 sites; the fastest
- * choice depends on shape and precision, which lym_autotune measures. */
+ * slices stay in its L2).  Every schedule and staging mode computes each link
+ * the same way, so the links agree bit for bit across them (the observables
+ * are the same site terms summed in another order); the two-row form agrees
+ * to FP32 rounding.  Default: COLUMN | LDS with 8192 sites; the fastest choice
+ * depends on shape and precision, which lym_autotune measures. */
 #define LYM_SCHED_LINEAR 0
 #define LYM_SCHED_CONTIG 1
 #define LYM_SCHED_COLUMN 2
 #define LYM_SCHED_LDS 16 /* or-ed into a kind: stage each 64-site run's own links in LDS */
+/* or-ed into a kind, LYM_PREC_MIXED only: while a flow call runs (lym_flow,
+ * lym_flow_steps, lym_profile_flow) the links are kept as their first two
+ * rows, row 2 = conj(row0 x row1) rebuilt where a kernel reads a link: a
+ * third fewer bytes per link, more arithmetic; the call converts the field at
+ * entry and back before it returns.  lym_autotune tries it per shape. */
+#define LYM_SCHED_TWO_ROW 32
 int lym_set_schedule(lym_ctx *ctx, int kind, int target);
 /* Times every candidate schedule on the context's own field (RK stage 1,
  * which leaves the field unchanged) and keeps the fastest; cache_path (may be

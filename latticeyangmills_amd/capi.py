@@ -16,6 +16,7 @@ ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "device out of me
 PRECISIONS = {"fp64": 0, "mixed": 1}
 SCHEDULES = {"linear": 0, "contig": 1, "column": 2}
 SCHED_LDS = 16  # or-ed into a schedule kind: stage each run's own links in LDS
+SCHED_TWO_ROW = 32  # or-ed into a schedule kind (mixed): the flow keeps links in two-row form
 CANDIDATES = {"su2": 0, "exph": 1}
 UNIFORMS_PER_HIT = {"su2": 13, "exph": 9}
 
@@ -191,8 +192,8 @@ class LatticeContext:
         return total.value, st
 
     # --- work schedule ----------------------------------------------------
-    def set_schedule(self, kind, target=0, lds=False):
-        code = SCHEDULES[kind] | (SCHED_LDS if lds else 0)
+    def set_schedule(self, kind, target=0, lds=False, two_row=False):
+        code = SCHEDULES[kind] | (SCHED_LDS if lds else 0) | (SCHED_TWO_ROW if two_row else 0)
         self._check(self._lib.lym_set_schedule(self._h, code, target), "lym_set_schedule")
 
     def autotune(self, cache_path=None):

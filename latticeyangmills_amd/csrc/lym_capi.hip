@@ -37,6 +37,8 @@ struct lym_ctx {
     hipStream_t stream = nullptr;
     void *W[2] = {nullptr, nullptr};  // double-buffered link field (SoA)
     int cur = 0;                      // W[cur] holds the current links
+    int rows = 3;                     // form of the links while the flow runs (2: two-row, SCHED_TWO_ROW)
+    int form = 3;                     // form of W[cur] now: 3 between calls
     bool eo = false;                  // ... in even-odd order (left so by lym_metropolis)
     void *acc = nullptr;              // RK accumulator, 8 reals per link
     ksum *partials = nullptr;         // per-workgroup observables partials
@@ -169,6 +171,7 @@ int apply_schedule(lym_ctx *c, int kind, int target)
     }
     c->nblk_obs = int(nblk);
     c->sched = std::move(h);
+    c->rows = (c->sched.kind & lym::SCHED_TWO_ROW) ? 2 : 3;
     return LYM_OK;
 }
 
@@ -188,8 +191,33 @@ int stage_lds_mode(const lym_ctx *c)
 template <typename S, int STAGE>
 void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *acc, double eps)
 {
+    if constexpr (sizeof(S) == 4)
+        if (c->form == 2) {
+            lym::launch_flow_stage_r2<STAGE>(stage_lds_mode(c), stencil_blocks(c), c->stream, win, wout, acc, c->g,
+                                             dev_sched(c), eps);
+            return;
+        }
     lym::launch_flow_stage<S, STAGE>(stage_lds_mode(c), stencil_blocks(c), c->stream, win, wout, acc, c->g,
                                      dev_sched(c), eps);
+}
+
+// Puts W[cur] into form `rows` (3: three-row, 2: two-row, the mixed flow's
+// SCHED_TWO_ROW) by a conversion into W[1 - cur]; the flow entry points take
+// the field into the schedule's form and back to three rows before returning.
+int to_form(lym_ctx *c, int rows)
+{
+    if (c->form == rows) return LYM_OK;
+    const dim3 grid(blocks_for(c->g.V, 256), 4);
+    auto *in = static_cast<const float2 *>(c->W[c->cur]);
+    auto *out = static_cast<float2 *>(c->W[1 - c->cur]);
+    if (rows == 2)
+        lym::k_to_rows2<float><<<grid, 256, 0, c->stream>>>(in, out, c->g.V);
+    else
+        lym::k_from_rows2<float><<<grid, 256, 0, c->stream>>>(in, out, c->g.V);
+    LYM_LAUNCHED(c, "k_to_rows2 / k_from_rows2");
+    c->cur ^= 1;
+    c->form = rows;
+    return LYM_OK;
 }
 
 template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
@@ -220,7 +248,10 @@ template <typename S> int enqueue_observables(lym_ctx *c, int slot, hipEvent_t *
 {
     const lym::F2<S> *f = static_cast<const lym::F2<S> *>(c->W[c->cur]);
     if (ev) LYM_HIP(c, hipEventRecord(ev[0], c->stream));
-    lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
+    if (sizeof(S) == 4 && c->form == 2)
+        lym::k_observables<S, 2><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
+    else
+        lym::k_observables<S><<<c->nblk_obs, 192, 0, c->stream>>>(f, c->g, dev_sched(c), c->partials);
     if (ev) LYM_HIP(c, hipEventRecord(ev[1], c->stream));
     lym::k_reduce_partials<<<1, lym::kReduceThreads, 0, c->stream>>>(c->partials, c->nblk_obs, c->obs + 4 * slot);
     LYM_LAUNCHED(c, "k_observables");
@@ -466,11 +497,13 @@ int lym_flow_steps(lym_ctx *c, double eps, int nsteps)
     if (!c || nsteps < 0 || !(eps > 0)) return c ? fail(c, LYM_ERR_ARG, "bad eps or nsteps") : LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
     if (int rc = to_lex(c)) return rc;
+    if (nsteps == 0) return LYM_OK;
+    if (int rc = to_form(c, c->rows)) return rc;
     for (int i = 0; i < nsteps; i++) {
         int rc = step(c, eps);
         if (rc) return rc;
     }
-    return LYM_OK;
+    return to_form(c, 3);
 }
 
 int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *series, int max_rows, int *rows_out)
@@ -485,12 +518,13 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
     int rc = to_lex(c);
     if (rc || (rc = ensure_slots(c, nmeas))) return rc;
     auto enqueue = [&]() -> int {
-        int slot = 0, r = observables(c, slot++);
+        int slot = 0, r = to_form(c, c->rows);
+        if (!r) r = observables(c, slot++);
         for (int t = 1; t < rows && !r; t++) {
             r = step(c, eps);
             if (!r && t % measure_every == 0) r = observables(c, slot++);
         }
-        return r;
+        return r ? r : to_form(c, 3);
     };
     if (c->use_graph) {
         // key: the flow parameters (the buffers are fixed while the graph lives)
@@ -506,6 +540,7 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(c->stream, &g);
             c->cur = k;  // capture enqueued nothing: the field is where it was
+            c->form = 3;
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
                 return rc;
@@ -517,7 +552,8 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             c->graph_key[k] = key;
         }
         LYM_HIP(c, hipGraphLaunch(c->graph[k], c->stream));
-        if ((rows - 1) & 1) c->cur = 1 - k;
+        // rows - 1 steps, plus the two form conversions of the two-row flow
+        if (((rows - 1) + (c->rows == 2 ? 2 : 0)) & 1) c->cur = 1 - k;
     } else if ((rc = enqueue())) {
         return rc;
     }
@@ -659,7 +695,7 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
     const int nmeas = measure_every ? nsteps / measure_every : 0;
     const size_t nev = 2 + 4 * size_t(nsteps) + 2 * size_t(nmeas);
     int rc = to_lex(c);
-    if (rc || (rc = ensure_events(c, nev))) return rc;
+    if (rc || (rc = ensure_events(c, nev)) || (rc = to_form(c, c->rows))) return rc;
     hipEvent_t *ev = c->events.data();
     LYM_HIP(c, hipEventRecord(ev[0], c->stream));
     size_t k = 2;
@@ -674,6 +710,7 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
         }
     }
     LYM_HIP(c, hipEventRecord(ev[1], c->stream));
+    if ((rc = to_form(c, 3))) return rc;
     LYM_HIP(c, hipEventSynchronize(ev[1]));
     float ms = 0;
     LYM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -701,9 +738,12 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
 int lym_set_schedule(lym_ctx *c, int kind, int target)
 {
     if (!c) return LYM_ERR_ARG;
-    const int order = kind & ~lym::SCHED_LDS;
-    if ((kind & ~(lym::SCHED_LDS | 3)) || order > lym::SCHED_COLUMN || (order == lym::SCHED_COLUMN && target < 1))
+    const int order = kind & 3;
+    if ((kind & ~(lym::SCHED_LDS | lym::SCHED_TWO_ROW | 3)) || order > lym::SCHED_COLUMN ||
+        (order == lym::SCHED_COLUMN && target < 1))
         return fail(c, LYM_ERR_ARG, "bad schedule");
+    if ((kind & lym::SCHED_TWO_ROW) && c->prec != LYM_PREC_MIXED)
+        return fail(c, LYM_ERR_ARG, "the two-row form is for the mixed path (FP64 multiplies out all rows)");
     LYM_HIP(c, hipSetDevice(c->device));
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return apply_schedule(c, kind, target);
@@ -734,17 +774,39 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
         source = "measured";
         int rc = ensure_events(c, 2);
         if (rc) return rc;
-        for (auto cand : lym::schedule_candidates()) {
+        const bool mixed = c->prec == LYM_PREC_MIXED;
+        // The two-row candidates read a two-row copy of the field and write a
+        // scratch field; both live only while tuning (the field stays as it is).
+        struct Scratch {
+            float2 *p[2] = {nullptr, nullptr};
+            ~Scratch()
+            {
+                for (float2 *q : p)
+                    if (q) (void)hipFree(q);
+            }
+        } r2;
+        if (mixed) {
+            for (auto &q : r2.p) LYM_HIP(c, hipMalloc(&q, sizeof(float2) * 24 * size_t(c->g.V)));
+            lym::k_to_rows2<float><<<dim3(blocks_for(c->g.V, 256), 4), 256, 0, c->stream>>>(
+                static_cast<const float2 *>(c->W[c->cur]), r2.p[0], c->g.V);
+            LYM_LAUNCHED(c, "k_to_rows2");
+        }
+        for (auto cand : lym::schedule_candidates(mixed)) {
             if ((rc = apply_schedule(c, cand.first, cand.second))) return rc;
             float best_c = 1e30f;
             for (int rep = 0; rep < 4; rep++) {  // stage 1 leaves W[cur] unchanged
                 LYM_HIP(c, hipEventRecord(c->events[0], c->stream));
-                if (c->prec == LYM_PREC_FP64)
+                if (c->prec == LYM_PREC_FP64) {
                     launch_stage<double, 1>(c, static_cast<double2 *>(c->W[c->cur]),
                                             static_cast<double2 *>(c->W[1 - c->cur]), static_cast<double *>(c->acc), 0.0);
-                else
+                } else if (c->rows == 2) {
+                    c->form = 2;
+                    launch_stage<float, 1>(c, r2.p[0], r2.p[1], static_cast<float *>(c->acc), 0.0);
+                    c->form = 3;
+                } else {
                     launch_stage<float, 1>(c, static_cast<float2 *>(c->W[c->cur]), static_cast<float2 *>(c->W[1 - c->cur]),
                                            static_cast<float *>(c->acc), 0.0);
+                }
                 LYM_LAUNCHED(c, "k_flow_stage");
                 LYM_HIP(c, hipEventRecord(c->events[1], c->stream));
                 LYM_HIP(c, hipEventSynchronize(c->events[1]));

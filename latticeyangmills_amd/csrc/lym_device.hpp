@@ -375,6 +375,53 @@ LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, unsigned s, const 
     }
 }
 
+// Links in two-row form (the mixed flow, chosen per lattice shape): a
+// direction plane holds the first two rows only, plane[k*V + s] for
+// k = 0..5, and row 2 = conj(row0 x row1) is rebuilt on load, in the
+// precision R (the links are SU(3): the mixed path reunitarises every link it
+// writes).  A third fewer bytes per link read and written; where a product
+// uses rows 0, 1 of a link only, the rebuild is dead code and drops out.
+template <int ROWS> constexpr int plane_elems() { return ROWS == 2 ? 6 : 9; }
+
+template <typename R, typename S, int ROWS = 3>
+LYM_DI su3<R> load_link_r(const vec2_t<S> *__restrict__ plane, long V, unsigned s)
+{
+    if constexpr (ROWS == 3) {
+        return load_link<R, S>(plane, V, s);
+    } else {
+        using F2 = vec2_t<S>;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 6 * V * long(sizeof(F2)));
+        const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+        su3<R> m;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const F2 v = buf_load<F2>(r, off, unsigned(k) * es);
+            m.e[k] = cplx<R>{R(v.x), R(v.y)};
+        }
+        complete_row2(m);
+        return m;
+    }
+}
+
+template <typename S, typename R, int AUX = 0, int ROWS = 3>
+LYM_DI void store_link_r(vec2_t<S> *__restrict__ plane, long V, unsigned s, const su3<R> &m)
+{
+    if constexpr (ROWS == 3) {
+        store_link<S, R, AUX>(plane, V, s, m);
+    } else {
+        using F2 = vec2_t<S>;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 6 * V * long(sizeof(F2)));
+        const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            F2 v;
+            v.x = S(m.e[k].re);
+            v.y = S(m.e[k].im);
+            buf_store<F2, AUX>(v, r, off, unsigned(k) * es);
+        }
+    }
+}
+
 // ------------------------------------------------------------- geometry ---
 
 // Lexicographic site index s = x + Lx*(y + Ly*(z + Lz*t)), periodic

@@ -36,13 +36,15 @@ template <typename S> using F2 = vec2_t<S>;
 // source decides from the tag alone whether the read comes from LDS, so no
 // lane ever branches on it.
 
-// Links of a field read straight from HBM (through L1/L2).
-template <typename S> struct GlobalLinks {
+// Links of a field read straight from HBM (through L1/L2).  ROWS: the
+// field's storage form, three rows or two (load_link_r).
+template <typename S, int ROWS = 3> struct GlobalLinks {
+    static constexpr int rows = ROWS;
     const F2<S> *f;
     long V;
     template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
-        return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+        return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
 
@@ -63,7 +65,8 @@ template <typename R, typename S> LYM_DI su3<R> lds_link(const F2<S> *p, int str
 // XROWS (runs hold whole x-rows, Lx | 64), come from LDS.  Per link update
 // 3 of the 19 staple reads are at the site itself and, with XROWS, another 3
 // are along x.
-template <typename S, bool XROWS> struct RunLinks {
+template <typename S, bool XROWS, int ROWS = 3> struct RunLinks {
+    static constexpr int rows = ROWS;
     const F2<S> *f;
     long V;
     long site0;
@@ -73,7 +76,7 @@ template <typename S, bool XROWS> struct RunLinks {
         if constexpr (ZT == 0 && DY == 0 && (DX == 0 || XROWS))
             return lds_link<R, S>(lds + dir * 9 * kSitesPerBlock + unsigned(site - site0), kSitesPerBlock);
         else
-            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+            return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
 
@@ -91,7 +94,8 @@ inline bool slab_fits(const int dims[4])
     const int lx = dims[0];
     return lx >= 2 && lx <= 32 && kSitesPerBlock % lx == 0 && dims[1] % (kSitesPerBlock / lx) == 0;
 }
-template <typename S> struct SlabLinks {
+template <typename S, int ROWS = 3> struct SlabLinks {
+    static constexpr int rows = ROWS;
     const F2<S> *f;
     long V;
     const F2<S> *slab;
@@ -102,7 +106,7 @@ template <typename S> struct SlabLinks {
             const int li = (row + 1 + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
             return lds_link<R, S>(slab + dir * 9 * kSlabSites + li, kSlabSites);
         } else {
-            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+            return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
         }
     }
 };
@@ -135,7 +139,8 @@ template <typename S> struct EOLinks {
 // Lx = 24) were measured slower than staging own links only.
 constexpr int kTileX = 16, kTileY = kSitesPerBlock / kTileX;
 inline bool tile_fits(const int dims[4]) { return dims[0] % kTileX == 0 && dims[1] % kTileY == 0; }
-template <typename S> struct TileLinks {
+template <typename S, int ROWS = 3> struct TileLinks {
+    static constexpr int rows = ROWS;
     const F2<S> *f;
     long V;
     const F2<S> *tile;
@@ -145,7 +150,7 @@ template <typename S> struct TileLinks {
         if constexpr (ZT == 0)
             return lds_link<R, S>(tile + dir * 9 * kSlabSites + self + DY * (kTileX + 2) + DX, kSlabSites);
         else
-            return load_link<R, S>(f + (long)dir * 9 * V, V, unsigned(site));
+            return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
 
@@ -260,7 +265,8 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     su3<double> E = (sizeof(S) == 4) ? exp_iq<kShortSeriesTermsF32, kSeriesTermsF32>(q) : exp_iq(q);
     su3<double> W = mul_nn(E, convert<double>(U));
     if constexpr (REUNIT) reunitarize(W);
-    store_link<S, double, kStreamAux>(wout + (long)MU * 9 * V, V, unsigned(s), W);
+    constexpr int ROWS = Src::rows;  // the output field takes the input's form
+    store_link_r<S, double, kStreamAux, ROWS>(wout + (long)MU * plane_elems<ROWS>() * V, V, unsigned(s), W);
 }
 
 // One 64-site run, all four directions (wave = direction).  LDS selects what
@@ -281,7 +287,7 @@ template <int LDS> constexpr int stage_lds_elems()
 {
     return LDS == 0 ? 1 : (LDS == 3 || LDS == 5) ? 4 * 9 * kSlabSites : 4 * 9 * kSitesPerBlock;
 }
-template <typename S, int STAGE, bool REUNIT, int LDS>
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
                        const Geom g, long site0, double eps, F2<S> *__restrict__ lds)
 {
@@ -303,11 +309,11 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         const long pb = (long)zt * Lx * Ly;
         const int lx = lane % TX, ly = lane / TX;
         const long s = pb + x0 + lx + (long)Lx * (y0 + ly);
-        const F2<S> *plane = win + (long)mu * 9 * g.V;
+        const F2<S> *plane = win + (long)mu * plane_elems<ROWS>() * g.V;
         F2<S> *dst = tile + mu * 9 * kSlabSites;
         const int self = (ly + 1) * W + lx + 1;
         {
-            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(s));
+            const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(s));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 dst[k * kSlabSites + self].x = u.e[k].re;
@@ -334,7 +340,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
             gx -= (gx >= Lx) ? Lx : 0;
             gy += (gy < 0) ? Ly : 0;
             gy -= (gy >= Ly) ? Ly : 0;
-            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(pb + gx + (long)Lx * gy));
+            const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(pb + gx + (long)Lx * gy));
             const int li = (hy + 1) * W + hx + 1;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
@@ -343,7 +349,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
             }
         }
         __syncthreads();
-        const TileLinks<S> src{win, g.V, tile, self};
+        const TileLinks<S, ROWS> src{win, g.V, tile, self};
         switch (mu) {
         case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
         case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
@@ -365,10 +371,10 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         const int y0 = int(uint32_t(site0) / uint32_t(Lx) % uint32_t(Ly));
         const long lo = (y0 > 0) ? site0 - Lx : site0 - Lx + long(Ly) * Lx;
         const long hi = (y0 + R < Ly) ? site0 + kSitesPerBlock : site0 + kSitesPerBlock - long(Ly) * Lx;
-        const F2<S> *plane = win + (long)mu * 9 * g.V;
+        const F2<S> *plane = win + (long)mu * plane_elems<ROWS>() * g.V;
         F2<S> *dst = slab + mu * 9 * kSlabSites;
         {
-            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(s));
+            const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(s));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 dst[k * kSlabSites + Lx + lane].x = u.e[k].re;
@@ -379,7 +385,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
             const bool below = lane < Lx;
             const long hs = below ? lo + lane : hi + (lane - Lx);
             const int li = below ? lane : (R + 1) * Lx + (lane - Lx);
-            const su3<S> u = load_link<S, S>(plane, g.V, unsigned(hs));
+            const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(hs));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 dst[k * kSlabSites + li].x = u.e[k].re;
@@ -387,7 +393,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
             }
         }
         __syncthreads();
-        const SlabLinks<S> src{win, g.V, slab, Lx - 1, lane >> __builtin_ctz(unsigned(Lx)), lane & (Lx - 1)};
+        const SlabLinks<S, ROWS> src{win, g.V, slab, Lx - 1, lane >> __builtin_ctz(unsigned(Lx)), lane & (Lx - 1)};
         switch (mu) {
         case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
         case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
@@ -397,7 +403,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         __syncthreads();
     } else if constexpr (LDS != 0) {
         if (active) {
-            const su3<S> u = load_link<S, S>(win + (long)mu * 9 * g.V, g.V, unsigned(s));
+            const su3<S> u = load_link_r<S, S, ROWS>(win + (long)mu * plane_elems<ROWS>() * g.V, g.V, unsigned(s));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 F2<S> v;
@@ -408,7 +414,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         }
         __syncthreads();
         if (active) {
-            const RunLinks<S, LDS == 2> src{win, g.V, site0, lds};
+            const RunLinks<S, LDS == 2, ROWS> src{win, g.V, site0, lds};
             switch (mu) {
             case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
             case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
@@ -419,7 +425,7 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         __syncthreads();
     } else {
         if (!active) return;
-        const GlobalLinks<S> src{win, g.V};
+        const GlobalLinks<S, ROWS> src{win, g.V};
         switch (mu) {
         case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
         case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
@@ -429,14 +435,14 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
     }
 }
 
-template <typename S, int STAGE, bool REUNIT, int LDS>
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
 __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                     S *__restrict__ acc, Geom g, Sched sc, double eps)
 {
     const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
     if (run < 0) return;
     __shared__ F2<S> lds[stage_lds_elems<LDS>()];
-    flow_block<S, STAGE, REUNIT, LDS>(win, wout, acc, g, run * kSitesPerBlock, eps, lds);
+    flow_block<S, STAGE, REUNIT, LDS, ROWS>(win, wout, acc, g, run * kSitesPerBlock, eps, lds);
 }
 
 // ----------------------------------------------------------- observables ---
@@ -448,11 +454,12 @@ __global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ wi
 // path) each leaf, a product of four SU(3) links, is formed from its first two
 // rows and completed by unitarity (240 instead of 324 operations per leaf);
 // FP64 multiplies out all three rows as the reference does.
-template <typename R, typename S, int MU, int NU>
+template <typename R, typename S, int MU, int NU, int ROWS = 3>
 LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, R *plaq)
 {
-    const F2<S> *Fm = f + (long)MU * 9 * V;
-    const F2<S> *Fn = f + (long)NU * 9 * V;
+    const F2<S> *Fm = f + (long)MU * plane_elems<ROWS>() * V;
+    const F2<S> *Fn = f + (long)NU * plane_elems<ROWS>() * V;
+    auto load_link = [](const F2<S> *plane, long V, long site) { return load_link_r<R, S, ROWS>(plane, V, unsigned(site)); };
     constexpr bool TWO_ROW = sizeof(R) == 4;
     auto add = [](su3<R> &cl, const su3<R> &t, R sign) {  // sign = +-1: one rounding, as cl +- t
 #pragma unroll
@@ -463,14 +470,14 @@ LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, 
     {  // P = ((U_mu(x) U_nu(x+mu)) U_mu(x+nu)^+) U_nu(x)^+
         if constexpr (TWO_ROW) {
             su3<R> t, u;
-            mul_nn_rows01(t, load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
-            mul_na_rows01(u, t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
-            mul_na_rows01(cl, u, load_link<R, S>(Fn, V, s));
+            mul_nn_rows01(t, load_link(Fm, V, s), load_link(Fn, V, s + n.fwd[MU]));
+            mul_na_rows01(u, t, load_link(Fm, V, s + n.fwd[NU]));
+            mul_na_rows01(cl, u, load_link(Fn, V, s));
             complete_row2(cl);
         } else {
-            su3<R> t = mul_nn(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU]));
-            t = mul_na(t, load_link<R, S>(Fm, V, s + n.fwd[NU]));
-            cl = mul_na(t, load_link<R, S>(Fn, V, s));
+            su3<R> t = mul_nn(load_link(Fm, V, s), load_link(Fn, V, s + n.fwd[MU]));
+            t = mul_na(t, load_link(Fm, V, s + n.fwd[NU]));
+            cl = mul_na(t, load_link(Fn, V, s));
         }
         *plaq = retrace(cl);
     }
@@ -479,14 +486,14 @@ LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, 
         su3<R> t;
         if constexpr (TWO_ROW) {
             su3<R> p, u;
-            mul_nn_cols01(p, load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
-            mul_an_rows01(u, p, load_link<R, S>(Fm, V, y));
-            mul_nn_rows01(t, u, load_link<R, S>(Fn, V, s + n.bwd[NU]));
+            mul_nn_cols01(p, load_link(Fn, V, y), load_link(Fm, V, s + n.bwd[MU]));
+            mul_an_rows01(u, p, load_link(Fm, V, y));
+            mul_nn_rows01(t, u, load_link(Fn, V, s + n.bwd[NU]));
             complete_row2(t);
         } else {
-            t = mul_nn(load_link<R, S>(Fn, V, y), load_link<R, S>(Fm, V, s + n.bwd[MU]));
-            t = mul_an(t, load_link<R, S>(Fm, V, y));
-            t = mul_nn(t, load_link<R, S>(Fn, V, s + n.bwd[NU]));
+            t = mul_nn(load_link(Fn, V, y), load_link(Fm, V, s + n.bwd[MU]));
+            t = mul_an(t, load_link(Fm, V, y));
+            t = mul_nn(t, load_link(Fn, V, s + n.bwd[NU]));
         }
         add(cl, t, R(1));
     }
@@ -495,14 +502,14 @@ LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, 
         su3<R> t;
         if constexpr (TWO_ROW) {
             su3<R> p, u;
-            mul_na_rows01(p, load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
-            mul_na_rows01(u, p, load_link<R, S>(Fm, V, y));
-            mul_nn_rows01(t, u, load_link<R, S>(Fn, V, y));
+            mul_na_rows01(p, load_link(Fm, V, s), load_link(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
+            mul_na_rows01(u, p, load_link(Fm, V, y));
+            mul_nn_rows01(t, u, load_link(Fn, V, y));
             complete_row2(t);
         } else {
-            t = mul_na(load_link<R, S>(Fm, V, s), load_link<R, S>(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
-            t = mul_na(t, load_link<R, S>(Fm, V, y));
-            t = mul_nn(t, load_link<R, S>(Fn, V, y));
+            t = mul_na(load_link(Fm, V, s), load_link(Fn, V, s + n.fwd[MU] + n.bwd[NU]));
+            t = mul_na(t, load_link(Fm, V, y));
+            t = mul_nn(t, load_link(Fn, V, y));
         }
         add(cl, t, R(-1));
     }
@@ -511,14 +518,14 @@ LYM_DI ah8<R> clover(const F2<S> *__restrict__ f, long V, long s, const Nbr &n, 
         su3<R> t;
         if constexpr (TWO_ROW) {
             su3<R> p, u;
-            mul_an_rows01(p, load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
-            mul_nn_rows01(u, p, load_link<R, S>(Fm, V, y + n.fwd[NU]));
-            mul_na_rows01(t, u, load_link<R, S>(Fn, V, s));
+            mul_an_rows01(p, load_link(Fm, V, y), load_link(Fn, V, y));
+            mul_nn_rows01(u, p, load_link(Fm, V, y + n.fwd[NU]));
+            mul_na_rows01(t, u, load_link(Fn, V, s));
             complete_row2(t);
         } else {
-            t = mul_an(load_link<R, S>(Fm, V, y), load_link<R, S>(Fn, V, y));
-            t = mul_nn(t, load_link<R, S>(Fm, V, y + n.fwd[NU]));
-            t = mul_na(t, load_link<R, S>(Fn, V, s));
+            t = mul_an(load_link(Fm, V, y), load_link(Fn, V, y));
+            t = mul_nn(t, load_link(Fm, V, y + n.fwd[NU]));
+            t = mul_na(t, load_link(Fn, V, s));
         }
         add(cl, t, R(-1));
     }
@@ -551,7 +558,7 @@ template <typename R> LYM_DI R retr_ah(const ah8<R> &A, const ah8<R> &B)
 // both planes, out[1] = Re Tr(C C) over both planes (the reference's raw
 // energy terms, superobs.cpp:109), out[2] = this pair's term of
 // q(x) = -[ReTr(C01 C23) + ReTr(C02 C13^+) + ReTr(C03 C12)] (superobs.cpp:113-124).
-template <typename R, typename S, int PAIR>
+template <typename R, typename S, int PAIR, int ROWS = 3>
 LYM_DI void pair_observables(const F2<S> *__restrict__ f, const Geom g, long s, double out[3])
 {
     constexpr int M1 = 0, N1 = PAIR + 1;
@@ -559,8 +566,8 @@ LYM_DI void pair_observables(const F2<S> *__restrict__ f, const Geom g, long s, 
     const long V = g.V;
     Nbr n = neighbours(g, s);
     R p1, p2;
-    ah8<R> A = clover<R, S, M1, N1>(f, V, s, n, &p1);
-    ah8<R> B = clover<R, S, M2, N2>(f, V, s, n, &p2);
+    ah8<R> A = clover<R, S, M1, N1, ROWS>(f, V, s, n, &p1);
+    ah8<R> B = clover<R, S, M2, N2, ROWS>(f, V, s, n, &p2);
     out[0] = double(p1) + double(p2);
     out[1] = double(retr_ah(A, A)) + double(retr_ah(B, B));
     const double t = double(retr_ah(A, B));
@@ -593,7 +600,7 @@ template <int NT> LYM_DI void block_ksum4(ksum v[4], ksum *out)
 // sum |q terms|}.
 constexpr int kObsSites = 64;
 
-template <typename S>
+template <typename S, int ROWS = 3>
 __global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f, Geom g, Sched sc,
                                                      ksum *__restrict__ partials)
 {
@@ -604,11 +611,11 @@ __global__ __launch_bounds__(192) void k_observables(const F2<S> *__restrict__ f
     if (run >= 0 && s < g.V) {
         double o[3];
         if (pair == 0)
-            pair_observables<S, S, 0>(f, g, s, o);
+            pair_observables<S, S, 0, ROWS>(f, g, s, o);
         else if (pair == 1)
-            pair_observables<S, S, 1>(f, g, s, o);
+            pair_observables<S, S, 1, ROWS>(f, g, s, o);
         else
-            pair_observables<S, S, 2>(f, g, s, o);
+            pair_observables<S, S, 2, ROWS>(f, g, s, o);
         v[0].s = o[0];
         v[1].s = o[1];
         v[2].s = o[2];
@@ -689,6 +696,29 @@ static __global__ __launch_bounds__(256) void k_check_links(const F2<S> *__restr
         atomicMax(out, (unsigned long long)__double_as_longlong(dev));
         if (nbad) atomicAdd(out + 1, nbad);
     }
+}
+
+// Three-row field -> two-row field (the first two rows of every link);
+// blockIdx.y = direction.
+template <typename S>
+__global__ __launch_bounds__(256) void k_to_rows2(const F2<S> *__restrict__ in, F2<S> *__restrict__ out, long V)
+{
+    const long s = (long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= V) return;
+    const long mu = blockIdx.y;
+#pragma unroll
+    for (int k = 0; k < 6; k++) out[(mu * 6 + k) * V + s] = in[(mu * 9 + k) * V + s];
+}
+
+// Two-row field -> three-row field, row 2 = conj(row0 x row1) formed in FP64.
+template <typename S>
+__global__ __launch_bounds__(256) void k_from_rows2(const F2<S> *__restrict__ in, F2<S> *__restrict__ out, long V)
+{
+    const long s = (long)blockIdx.x * 256 + threadIdx.x;
+    if (s >= V) return;
+    const long mu = blockIdx.y;
+    const su3<double> m = load_link_r<double, S, 2>(in + mu * 6 * V, V, unsigned(s));
+    store_link<S, double>(out + mu * 9 * V, V, unsigned(s), m);
 }
 
 // reference layout (V x 18 doubles for one mu) -> SoA plane of direction mu

@@ -1,8 +1,8 @@
 // Host-side launchers of the heavy kernel families: the RK stage kernels of
 // the single-lattice and batch paths and the Metropolis sub-sweep kernels.
 // Each family is instantiated for one link precision per translation unit
-// (lym_stage_*.hip: single lattice, lym_bstage_*.hip: batch,
-// lym_metro_*.hip: Metropolis; f64 and f32), so the device code of the
+// (lym_stage_*.hip: single lattice, lym_stage_f32r2.hip: its two-row form,
+// lym_bstage_*.hip: batch, lym_metro_*.hip: Metropolis; f64 and f32), so the device code of the
 // library compiles in parallel; every other translation unit sees the extern
 // declarations below.
 #pragma once
@@ -23,6 +23,21 @@ void launch_flow_stage(int lds, unsigned nb, hipStream_t st, const F2<S> *win, F
     case 2: k_flow_stage<S, STAGE, re, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
     case 3: k_flow_stage<S, STAGE, re, 3><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
     default: k_flow_stage<S, STAGE, re, 5><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    }
+}
+
+// One RK stage of the mixed path over a field in two-row form (ROWS = 2);
+// lds as launch_flow_stage.
+template <int STAGE>
+void launch_flow_stage_r2(int lds, unsigned nb, hipStream_t st, const F2<float> *win, F2<float> *wout, float *acc,
+                          Geom g, Sched sc, double eps)
+{
+    switch (lds) {
+    case 0: k_flow_stage<float, STAGE, true, 0, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 1: k_flow_stage<float, STAGE, true, 1, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 2: k_flow_stage<float, STAGE, true, 2, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    case 3: k_flow_stage<float, STAGE, true, 3, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
+    default: k_flow_stage<float, STAGE, true, 5, 2><<<nb, 256, 0, st>>>(win, wout, acc, g, sc, eps); break;
     }
 }
 
@@ -98,6 +113,14 @@ LYM_STAGE_EXTERN(double, 3)
 LYM_STAGE_EXTERN(float, 1)
 LYM_STAGE_EXTERN(float, 2)
 LYM_STAGE_EXTERN(float, 3)
+#endif
+#ifndef LYM_INSTANTIATE_STAGES_R2
+extern template void launch_flow_stage_r2<1>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *, Geom,
+                                             Sched, double);
+extern template void launch_flow_stage_r2<2>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *, Geom,
+                                             Sched, double);
+extern template void launch_flow_stage_r2<3>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *, Geom,
+                                             Sched, double);
 #endif
 #ifndef LYM_INSTANTIATE_METRO
 LYM_METRO_EXTERN(double)

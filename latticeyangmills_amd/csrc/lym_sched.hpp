@@ -27,6 +27,9 @@ namespace lym {
 enum SchedKind { SCHED_LINEAR = 0, SCHED_CONTIG = 1, SCHED_COLUMN = 2 };
 // flag or-ed into a kind: the stage kernel stages each run's own links in LDS
 constexpr int SCHED_LDS = 16;
+// flag or-ed into a kind (mixed precision, single lattice): the flow keeps the
+// links in two-row form while it runs
+constexpr int SCHED_TWO_ROW = 32;
 constexpr int kXcds = 8;
 constexpr int kRunSites = 64;
 
@@ -93,7 +96,7 @@ inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, 
     HostSchedule h;
     h.kind = kind;
     h.target = target;
-    kind &= ~SCHED_LDS;
+    kind &= ~(SCHED_LDS | SCHED_TWO_ROW);
     if (kind == SCHED_LINEAR) return h;
     std::vector<int> all;
     for (size_t c = 0; c < dims.size(); c++) {
@@ -147,25 +150,35 @@ inline HostSchedule metro_column_schedule(const std::array<int, 4> &L, int targe
     return h;
 }
 
-// the candidates the autotuners time: every order, with and without LDS staging
-inline std::vector<std::pair<int, int>> schedule_candidates()
+// the candidates the autotuners time: every order, with and without LDS
+// staging, and (two_row: the mixed single-lattice path) each of these again
+// with the links in two-row form
+inline std::vector<std::pair<int, int>> schedule_candidates(bool two_row = false)
 {
     std::vector<std::pair<int, int>> c;
-    for (int lds : {0, SCHED_LDS})
-        for (auto k : std::vector<std::pair<int, int>>{{SCHED_LINEAR, 0},
-                                                       {SCHED_CONTIG, 0},
-                                                       {SCHED_COLUMN, 256},
-                                                       {SCHED_COLUMN, 512},
-                                                       {SCHED_COLUMN, 1024},
-                                                       {SCHED_COLUMN, 2048},
-                                                       {SCHED_COLUMN, 4096},
-                                                       {SCHED_COLUMN, 8192}})
-            c.push_back({k.first | lds, k.second});
+    const std::vector<std::pair<int, int>> orders{{SCHED_LINEAR, 0},    {SCHED_CONTIG, 0},    {SCHED_COLUMN, 256},
+                                                  {SCHED_COLUMN, 512},  {SCHED_COLUMN, 1024}, {SCHED_COLUMN, 2048},
+                                                  {SCHED_COLUMN, 4096}, {SCHED_COLUMN, 8192}};
+    for (int form : {0, SCHED_TWO_ROW}) {
+        if (form && !two_row) break;
+        for (int lds : {0, SCHED_LDS})
+            for (auto k : orders) c.push_back({k.first | lds | form, k.second});
+    }
     return c;
 }
 
 inline const char *schedule_name(int kind)
 {
+    if (kind & SCHED_TWO_ROW) {
+        switch (kind & ~SCHED_TWO_ROW) {
+        case SCHED_LINEAR: return "linear+2row";
+        case SCHED_CONTIG: return "contig+2row";
+        case SCHED_COLUMN: return "column+2row";
+        case SCHED_LINEAR | SCHED_LDS: return "linear+lds+2row";
+        case SCHED_CONTIG | SCHED_LDS: return "contig+lds+2row";
+        default: return "column+lds+2row";
+        }
+    }
     switch (kind) {
     case SCHED_LINEAR: return "linear";
     case SCHED_CONTIG: return "contig";

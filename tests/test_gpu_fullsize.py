@@ -179,7 +179,8 @@ def test_32x32x32x64_mixed_within_stated_error():
 def test_48x48x48x96_mixed_within_stated_error():
     """48^3x96, beta 6.4: 200 Philox sweeps (cold start, seed 64), then the
     mixed and the FP64 flow of that config, eps 0.02 to t = 7.98, measured
-    every 5 steps.  Bounds at every measured row: <E> and the plaquette 1e-6
+    every 5 steps, the mixed flow with the links in three-row and in two-row
+    form.  Bounds at every measured row: <E> and the plaquette 1e-6
     relative, |dQ| <= 1e-4; t0/a^2 and w0/a 1e-6 relative where the flow
     reaches them; links of the mixed path within 1e-6 of SU(3)."""
     from latticeyangmills_amd import scale
@@ -189,22 +190,28 @@ def test_48x48x48x96_mixed_within_stated_error():
         ctx.metropolis(6.4, 0.24, 10, 200, 0, "exph", 64)
         links = ctx.download()
         s64 = ctx.flow(0.02, 8.0, 5)
+    results = {}
     with LatticeContext(dims, "mixed") as ctx:
-        ctx.upload(links)
-        del links
-        smx = ctx.flow(0.02, 8.0, 5)
-        dev, bad = ctx.check_links()
-    assert s64.shape == smx.shape == (400, 4)
+        for form in ("three-row", "two-row"):
+            if form == "two-row":  # the links as their first two rows while the flow runs
+                ctx.set_schedule("column", 1024, True, two_row=True)
+            ctx.upload(links)
+            results[form] = (ctx.flow(0.02, 8.0, 5), ctx.check_links())
+    del links
+    assert s64.shape == (400, 4)
     rows = range(0, 400, 5)
-    e_err = max(rel(smx[r, 3], s64[r, 3]) for r in rows)
-    q_err = max(abs(smx[r, 2] - s64[r, 2]) for r in rows)
-    p_err = max(rel(smx[r, 1], s64[r, 1]) for r in rows)
-    print(f"48^3x96 mixed vs FP64: max rel dE {e_err:.2e}, max |dQ| {q_err:.2e}, max rel dplaq {p_err:.2e}")
-    assert e_err < 1e-6 and q_err < 1e-4 and p_err < 1e-6
+    for form, (smx, (dev, bad)) in results.items():
+        assert smx.shape == (400, 4)
+        e_err = max(rel(smx[r, 3], s64[r, 3]) for r in rows)
+        q_err = max(abs(smx[r, 2] - s64[r, 2]) for r in rows)
+        p_err = max(rel(smx[r, 1], s64[r, 1]) for r in rows)
+        print(f"48^3x96 mixed ({form}) vs FP64: max rel dE {e_err:.2e}, max |dQ| {q_err:.2e}, "
+              f"max rel dplaq {p_err:.2e}")
+        assert e_err < 1e-6 and q_err < 1e-4 and p_err < 1e-6, form
+        t0m, t064 = scale.t0(smx, V), scale.t0(s64, V)
+        assert (t0m is None) == (t064 is None) and (t064 is None or rel(t0m, t064) < 1e-6), form
+        w0m, w064 = scale.w0(smx, V), scale.w0(s64, V)
+        assert (w0m is None) == (w064 is None) and (w064 is None or rel(w0m, w064) < 1e-6), form
+        assert bad == 0 and dev < 1e-6, form
     m = s64[::5]
     assert np.all(np.diff(m[:, 1]) > 0)  # the plaquette rises along the flow
-    t0m, t064 = scale.t0(smx, V), scale.t0(s64, V)
-    assert (t0m is None) == (t064 is None) and (t064 is None or rel(t0m, t064) < 1e-6)
-    w0m, w064 = scale.w0(smx, V), scale.w0(s64, V)
-    assert (w0m is None) == (w064 is None) and (w064 is None or rel(w0m, w064) < 1e-6)
-    assert bad == 0 and dev < 1e-6

@@ -417,3 +417,59 @@ def test_staples_large_lattice_against_small_path():
     want_z = orc.derivative(links, dims, 3)
     assert np.max(np.abs(om - want_om)) < 1e-12
     assert np.max(np.abs(z - want_z)) < 1e-12
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 2, 2), (32, 4, 2, 2), (48, 4, 2, 2), (6, 4, 4, 8), (16, 8, 2, 4)])
+@pytest.mark.parametrize("kind,lds", [("linear", False), ("linear", True), ("column", True)])
+def test_two_row_form_flow(dims, kind, lds):
+    """Mixed path with the links in two-row form while the flow runs (row 2
+    rebuilt by unitarity where read; slab, tile, own-link and no staging):
+    one step within FP32 rounding of the oracle, and a 5-step series within
+    the mixed path's bounds of the three-row mixed path; the field comes back
+    in three-row form."""
+    links = orc.hot_start(dims, 61)
+    want = orc.flow_step(links, dims, 0.02)
+    target = 256 if kind == "column" else 0
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.set_schedule(kind, target, lds, two_row=True)
+        ctx.upload(links)
+        ctx.flow_steps(0.02, 1)
+        got = ctx.download()
+        assert np.max(np.abs(got - want)) < 2e-6
+        ctx.upload(links)
+        s2 = ctx.flow(0.02, 0.12, 1)
+        dev, bad = ctx.check_links()
+        assert bad == 0 and dev < 1e-6
+    with LatticeContext(dims, "mixed") as ctx:
+        ctx.set_schedule(kind, target, lds)
+        ctx.upload(links)
+        s3 = ctx.flow(0.02, 0.12, 1)
+    scale = orc.observables(links, dims)[3]
+    assert np.array_equal(s2[:, 0], s3[:, 0])
+    assert np.all(np.abs(s2[:, [1, 3]] / s3[:, [1, 3]] - 1) < 1e-6)
+    assert np.all(np.abs(s2[:, 2] - s3[:, 2]) < 1e-6 * max(scale, 1.0))
+
+
+def test_two_row_form_graph_replays_and_tuning():
+    """The two-row flow replayed from its graph equals its eager launches bit
+    for bit (the form conversions are part of the graph); the tuner, which
+    times two-row candidates on a scratch copy, leaves the field as it was."""
+    dims = (16, 8, 4, 4)
+    links = orc.hot_start(dims, 62)
+    with LatticeContext(dims, "mixed") as a, LatticeContext(dims, "mixed") as b:
+        for ctx in (a, b):
+            ctx.set_schedule("column", 512, True, two_row=True)
+        b.set_graph(True)
+        for rep in range(3):
+            a.upload(links)
+            b.upload(links)
+            want, got = a.flow(0.02, 0.14, 2), b.flow(0.02, 0.14, 2)
+            assert np.array_equal(np.isnan(want), np.isnan(got))
+            assert np.array_equal(want[~np.isnan(want)], got[~np.isnan(got)]), rep
+            np.testing.assert_array_equal(b.download(), a.download())
+        a.upload(links)
+        summary = a.autotune(None)
+        assert "measured" in summary
+        b.set_graph(False)
+        b.upload(links)
+        np.testing.assert_array_equal(a.download(), b.download())

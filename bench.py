@@ -36,15 +36,21 @@ STAGE_REALS = (44, 52, 44)
 # VALU arithmetic per link-update (three RK stages), from the rocprofv3 VALU
 # instruction counters of the stage kernels at 32^3x64 (64 lanes x (ADD + MUL
 # + 2 FMA), FP32 and FP64 summed; per-stage figures in PROFILE_FILE): the
-# mixed path forms its staples from two rows (2.82 kflop per link per stage),
-# FP64 multiplies out all three (3.21 kflop).
-FLOP_PER_LINK_UPDATE = {"mixed": 8460, "fp64": 9620}
+# mixed path (two-row staples, links in two-row form with row 2 rebuilt where
+# read) 3.08 kflop per link per stage, FP64 (all three rows multiplied out)
+# 3.21 kflop.
+FLOP_PER_LINK_UPDATE = {"mixed": 9244, "fp64": 9631}
 # HBM-side bytes per k_flow_stage launch, averaged over the three RK stages,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this bench (fetch
 # scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
-# back); the run, its commands and the per-stage figures are in PROFILE_FILE.
+# back), with the schedules the autotuner picks; the run, its commands and the
+# per-stage figures are in PROFILE_FILE.  The mixed flow keeps its links as
+# two rows (12 reals), so it moves fewer bytes than the canonical 18-real
+# count of REALS_PER_LINK_UPDATE.
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
-PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): 3.359e9, ("mixed-32x64", "fp64"): 5.445e9}
+PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): (1.405e9, PROFILE_FILE), ("mixed-32x64", "fp64"): (6.412e9, PROFILE_FILE),
+                    ("mixed-48x96", "mixed"): (8.694e9, "profiles/k_flow_stage_48x48x48x96.md"),
+                    ("mixed-48x96", "fp64"): (25.60e9, "profiles/k_flow_stage_48x48x48x96.md")}
 
 WORKLOADS = {
     # 1000 sweeps from a Haar-random start put t0 (t^2 <E> = 0.3) near t = 5.1,
@@ -166,14 +172,15 @@ def cpu_ensemble_baseline(eps=0.01, tau=2.0, every=10, max_procs=16):
 BATCH_DIMS = [(8, 8, 8, 8)] * 16 + [(12, 12, 12, 12)] * 16 + [(16, 16, 16, 16)] * 16 + [(16, 16, 16, 32)] * 16
 
 
-def batched_ensemble(tune_cache, every=10):
+def batched_ensemble(tune_cache, every=10, use_graph=True):
     """64 uneven lattices (16 each of 8^4, 12^4, 16^4, 16^3x32), beta = 6.0,
     Haar hot starts with seeds 0..63, 100 Metropolis sweeps, then the mixed
     flow eps = 0.01 to t = 2.0 (199 steps, observables every `every` steps) as
     one persistent launch per RK stage, graph-captured, autotuned; against the
     64 lattices flowed one after another by the plain FP64 single-lattice path,
     and (to separate the batching gain from the precision gain) by the mixed
-    single-lattice path."""
+    single-lattice path.  use_graph=False launches the batch eagerly (for a
+    kernel trace of its launches one by one)."""
     from latticeyangmills_amd import LatticeContext
     from latticeyangmills_amd.batch import BatchContext
     eps, tau = 0.01, 2.0
@@ -186,11 +193,11 @@ def batched_ensemble(tune_cache, every=10):
         for c, L in enumerate(confs):
             b.upload(c, L)
         tuner = b.autotune(tune_cache)
-        b.flow(eps, tau, every, use_graph=True)  # captures the graph
+        b.flow(eps, tau, every, use_graph=use_graph)  # captures the graph
         for c, L in enumerate(confs):
             b.upload(c, L)
         t0 = time.perf_counter()
-        series_b = b.flow(eps, tau, every, use_graph=True)
+        series_b = b.flow(eps, tau, every, use_graph=use_graph)
         t_batch = time.perf_counter() - t0
     def sequential(precision):
         ctxs = []
@@ -214,7 +221,7 @@ def batched_ensemble(tune_cache, every=10):
     return {"configs_per_s": n / t_batch, "seconds": t_batch, "sequential_plain_fp64_configs_per_s": n / t_seq,
             "sequential_seconds": t_seq, "speedup": t_seq / t_batch, "tuner": tuner,
             "sequential_mixed_configs_per_s": n / t_seq_mixed, "speedup_vs_sequential_mixed": t_seq_mixed / t_batch,
-            "max_rel_err_E_vs_fp64": e_err,
+            "max_rel_err_E_vs_fp64": e_err, "graph": use_graph,
             "workload": "64 lattices: 16 x 8^4, 16 x 12^4, 16 x 16^4, 16 x 16^3x32; beta 6.0; Haar seeds 0..63; "
                         f"100 sweeps; mixed flow eps {eps} to t {tau} (199 steps), observables every {every}"}
 
@@ -230,6 +237,7 @@ def main():
     ap.add_argument("--sweeps", type=int, default=None, help="override the Metropolis sweep count")
     ap.add_argument("--calibrate", action="store_true", help="also time a 1 GiB streaming copy (HBM probe)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--batch-eager", action="store_true", help="launch the batched flow eagerly, not from its graph")
     ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: tune afresh)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -303,7 +311,7 @@ def main():
     bytes_step = 4 * V * REALS_PER_LINK_UPDATE * real
     # per launch, averaged over the three stage launches of a step
     achieved_gbs = bytes_step / (step_kernel_ms * 1e-3) / 1e9
-    traffic = PROFILED_TRAFFIC.get((args.workload, main_prec))
+    traffic, traffic_source = PROFILED_TRAFFIC.get((args.workload, main_prec), (None, None))
     out = {
         "metric": METRIC, "value": lups, "unit": "link-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -318,7 +326,8 @@ def main():
                    "measure_every": every, "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": PROFILE_FILE if traffic else None,
+                     "traffic_source": traffic_source,
+                     "traffic_gbs": traffic / (step_kernel_ms / 3 * 1e-3) / 1e9 if traffic else None,
                      "algorithmic_bytes_per_launch": bytes_step / 3,
                      "kernel": "k_flow_stage (RK stages 1+2+3 per step)",
                      "stage_ms": [float(x) for x in st[:3]], "observables_ms": float(st[3]),
@@ -354,7 +363,7 @@ def main():
             "t2E_at_tmax": s64[last, 0] ** 2 * scale.e_mean(s64[last, 3], V), "Q_at_tmax": s64[last, 2],
         }
     if not args.no_batch:
-        out["batched"] = batched_ensemble(args.tune_cache)
+        out["batched"] = batched_ensemble(args.tune_cache, use_graph=not args.batch_eager)
         if cpu_ens:
             out["batched"]["cpu_baseline"] = cpu_ens
         log(f"[bench] batched: {out['batched']}")

@@ -35,6 +35,10 @@ extern "C" {
 
 #define LYM_PREC_FP64 0  /* FP64 links, FP64 arithmetic throughout */
 #define LYM_PREC_MIXED 1 /* FP32 links/staples/clover; FP64 exp, reunitarisation, reductions */
+/* An FP32 field keeps rows 0, 1 of every link rounded to FP32 and row 2 =
+ * conj(row0 x row1) formed from them in FP32 (exact SU(3) completion, so the
+ * upload of an SU(3) link moves row 2 by FP32 rounding only; a non-unitary
+ * link keeps its rows 0, 1 and takes the derived row 2). */
 
 #define LYM_CAND_SU2 0  /* X = R*S*T of SU(2) subgroups, Random::randSU3Transf (src/Math/random.cpp:71-156) */
 #define LYM_CAND_EXPH 1 /* X = exp(i eps H), H = sum_a h_a lambda_a, h_a uniform in [-1, 1) */
@@ -55,7 +59,8 @@ int lym_info(const lym_ctx *ctx, int dims[4], int64_t *volume, int *precision);
 
 /* Host <-> device copies of the gauge field, reference layout (see top).
  * Upload replaces InputConf::readConf's fill of the GluonField
- * (src/InputOutput/inputconf.cpp:114-155). */
+ * (src/InputOutput/inputconf.cpp:114-155); in LYM_PREC_MIXED it stores the
+ * FP32 form above. */
 int lym_upload_links(lym_ctx *ctx, const double *const by_mu[4]);
 int lym_download_links(lym_ctx *ctx, double *const by_mu[4]);
 /* Cold start, all U = 1 (setToIdentity, src/Math/lattice.cpp:33-41). */
@@ -141,8 +146,8 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
  * about `target` sites of the (x, y, z) volume along t, so neighbouring time
  * slices stay in its L2).  Every schedule and staging mode computes each link
  * the same way, so the links agree bit for bit across them (the observables
- * are the same site terms summed in another order); the two-row form agrees
- * to FP32 rounding.  Default: COLUMN | LDS with 8192 sites; the fastest choice
+ * are the same site terms summed in another order), the two-row form
+ * included: it drops the row 2 that an FP32 field derives anyway.  Default: COLUMN | LDS with 8192 sites; the fastest choice
  * depends on shape and precision, which lym_autotune measures. */
 #define LYM_SCHED_LINEAR 0
 #define LYM_SCHED_CONTIG 1
@@ -173,7 +178,8 @@ int lym_autotune(lym_ctx *ctx, const char *cache_path, char *summary, int summar
  *           layout of the library: direction planes mu = 0..3, each 9
  *           complex elements e = 3i + j by V sites,
  *           links[((mu*9 + e)*V + s)] = (Re U_ij, Im U_ij) of site s, as
- *           double2 (LYM_PREC_FP64) or float2 (LYM_PREC_MIXED);
+ *           double2 (LYM_PREC_FP64) or float2 (LYM_PREC_MIXED), read as
+ *           given (a field a context wrote holds the FP32 form above);
  *   out:    4 doubles in device memory, written as lym_observables' out;
  *   stream: the hipStream_t to run on (NULL: the null stream); the call is
  *           asynchronous on it.

@@ -49,9 +49,10 @@ int lym_batch_observables(lym_batch *b, double *out);
 int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
                    int max_rows, int *rows_out);
 /* Launch shape of the stage kernel: the schedule kind and target as
- * lym_set_schedule (lattices stay whole, in order); grid = 0 means one
- * workgroup per schedule slot, otherwise a persistent grid of that many
- * workgroups (rounded up to a multiple of 8). */
+ * lym_set_schedule (lattices stay whole, in order; LYM_SCHED_TWO_ROW for a
+ * mixed batch keeps every lattice in two-row form while lym_batch_flow runs);
+ * grid = 0 means one workgroup per schedule slot, otherwise a persistent grid
+ * of that many workgroups (rounded up to a multiple of 8). */
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target);
 /* Times the candidate shapes on the batch itself and keeps the fastest;
  * cache_path (may be NULL) holds one line per batch shape and precision.

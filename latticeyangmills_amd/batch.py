@@ -3,7 +3,7 @@ import ctypes
 
 import numpy as np
 
-from .capi import CANDIDATES, LYM_OK, PRECISIONS, SCHED_LDS, SCHEDULES, LymError, _D, _dptr, _links_ptrs, lib
+from .capi import CANDIDATES, LYM_OK, PRECISIONS, SCHED_LDS, SCHED_TWO_ROW, SCHEDULES, LymError, _D, _dptr, _links_ptrs, lib
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -101,8 +101,8 @@ class BatchContext:
                                              series.shape[1], ctypes.byref(r)), "lym_batch_flow")
         return series[:, : r.value]
 
-    def set_schedule(self, grid=0, kind="column", target=4096, lds=False):
-        code = SCHEDULES[kind] | (SCHED_LDS if lds else 0)
+    def set_schedule(self, grid=0, kind="column", target=4096, lds=False, two_row=False):
+        code = SCHEDULES[kind] | (SCHED_LDS if lds else 0) | (SCHED_TWO_ROW if two_row else 0)
         self._check(self._lib.lym_batch_set_schedule(self._h, grid, code, target), "lym_batch_set_schedule")
 
     def autotune(self, cache_path=None):
@@ -116,5 +116,5 @@ class BatchContext:
         self._check(self._lib.lym_batch_info(self._h, ctypes.byref(n), ctypes.byref(sites), ctypes.byref(grid),
                                              ctypes.byref(kind), ctypes.byref(target)), "lym_batch_info")
         return dict(nconf=n.value, total_sites=sites.value, grid=grid.value,
-                    kind=[k for k, v in SCHEDULES.items() if v == kind.value & ~SCHED_LDS][0],
-                    lds=bool(kind.value & SCHED_LDS), target=target.value)
+                    kind=[k for k, v in SCHEDULES.items() if v == kind.value & 3][0],
+                    lds=bool(kind.value & SCHED_LDS), two_row=bool(kind.value & SCHED_TWO_ROW), target=target.value)

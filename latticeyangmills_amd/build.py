@@ -30,8 +30,9 @@ HIP_FLAGS = [
 # One object per translation unit, compiled in parallel: the stage and
 # Metropolis kernel families are instantiated per link precision in units of
 # their own (lym_launch.hpp), the C ABI of the two paths in lym_capi / lym_batch.
-LIB_SOURCES = ["lym_bstage_f64.hip", "lym_bstage_f32.hip", "lym_stage_f64.hip", "lym_stage_f32.hip", "lym_stage_f32r2.hip",
-               "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip", "lym_batch.hip", "lym_noalloc.hip"]
+LIB_SOURCES = ["lym_bstage_f64.hip", "lym_bstage_f32.hip", "lym_bstage_f32r2.hip", "lym_stage_f64.hip",
+               "lym_stage_f32.hip", "lym_stage_f32r2.hip", "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip",
+               "lym_batch.hip", "lym_noalloc.hip"]
 OBJ_DIR = os.path.join(PKG, "_build")
 
 

@@ -35,6 +35,8 @@ struct lym_batch {
     void *W[2] = {nullptr, nullptr};
     int cur = 0;
     bool eo = false;  // W[cur] in even-odd order (left so by lym_batch_metropolis)
+    int rows = 3;     // form of the links while the flow runs (2: two-row, SCHED_TWO_ROW)
+    int form = 3;     // form of W[cur] now: 3 between calls
     void *acc = nullptr;
     ksum *partials = nullptr;
     double *obs = nullptr;
@@ -105,6 +107,7 @@ int apply_schedule(lym_batch *b, int grid, int kind, int target)
     }
     b->sched = std::move(h);
     b->grid = grid > 0 ? ((grid + lym::kXcds - 1) / lym::kXcds) * lym::kXcds : 0;
+    b->rows = (b->sched.kind & lym::SCHED_TWO_ROW) ? 2 : 3;
     drop_graph(b);
     return LYM_OK;
 }
@@ -126,8 +129,37 @@ int batch_lds_mode(const lym_batch *b)
 template <typename S, int STAGE>
 void launch_stage(const lym_batch *b, const lym::F2<S> *win, lym::F2<S> *wout, S *acc, double eps)
 {
+    if constexpr (sizeof(S) == 4)
+        if (b->form == 2) {
+            lym::launch_batch_stage_r2<STAGE>(batch_lds_mode(b), stage_grid(b), b->stream, win, wout, acc, b->d_desc,
+                                              b->nconf, b->ntiles, dev_sched(b), eps);
+            return;
+        }
     lym::launch_batch_stage<S, STAGE>(batch_lds_mode(b), stage_grid(b), b->stream, win, wout, acc, b->d_desc, b->nconf,
                                       b->ntiles, dev_sched(b), eps);
+}
+
+// Every lattice of W[cur] into form `rows` (3 or 2, the mixed flow's
+// SCHED_TWO_ROW) by a conversion into W[1 - cur], as the single-lattice
+// to_form: a lattice's three-row field starts at link_off, its two-row field
+// at link_off / 9 * 6.
+int to_form(lym_batch *b, int rows)
+{
+    if (b->form == rows) return LYM_OK;
+    auto *in = static_cast<const float2 *>(b->W[b->cur]);
+    auto *out = static_cast<float2 *>(b->W[1 - b->cur]);
+    for (const ConfDesc &d : b->desc) {
+        const dim3 grid(nblocks(d.g.V, 256), 4);
+        const long off3 = d.link_off, off2 = d.link_off / 9 * 6;
+        if (rows == 2)
+            lym::k_to_rows2<float><<<grid, 256, 0, b->stream>>>(in + off3, out + off2, d.g.V);
+        else
+            lym::k_from_rows2<float><<<grid, 256, 0, b->stream>>>(in + off2, out + off3, d.g.V);
+    }
+    BLAUNCHED(b, "k_to_rows2 / k_from_rows2");
+    b->cur ^= 1;
+    b->form = rows;
+    return LYM_OK;
 }
 
 template <typename S> int enqueue_step(lym_batch *b, double eps)
@@ -151,8 +183,12 @@ int step(lym_batch *b, double eps)
 template <typename S> int enqueue_obs(lym_batch *b, int slot)
 {
     const unsigned g = unsigned(std::min<long>(b->ntiles, 65535L * 4));
-    lym::k_batch_observables<S><<<g, 192, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]), b->d_desc,
-                                                          b->nconf, b->ntiles, b->partials);
+    if (sizeof(S) == 4 && b->form == 2)
+        lym::k_batch_observables<S, 2><<<g, 192, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]),
+                                                                 b->d_desc, b->nconf, b->ntiles, b->partials);
+    else
+        lym::k_batch_observables<S><<<g, 192, 0, b->stream>>>(static_cast<const lym::F2<S> *>(b->W[b->cur]), b->d_desc,
+                                                              b->nconf, b->ntiles, b->partials);
     lym::k_batch_reduce<<<b->nconf, 256, 0, b->stream>>>(b->partials, b->d_desc, b->nconf, b->ntiles,
                                                         b->obs + size_t(4) * b->nconf * slot);
     BLAUNCHED(b, "k_batch_observables");
@@ -188,13 +224,13 @@ void normalise(const lym_batch *b, int c, const double *raw, double out[4])
 int enqueue_flow(lym_batch *b, double eps, int rows, int every)
 {
     int slot = 0, rc;
-    if ((rc = observables(b, slot++))) return rc;
+    if ((rc = to_form(b, b->rows)) || (rc = observables(b, slot++))) return rc;
     for (int t = 1; t < rows; t++) {
         if ((rc = step(b, eps))) return rc;
         if (t % every == 0)
             if ((rc = observables(b, slot++))) return rc;
     }
-    return LYM_OK;
+    return to_form(b, 3);
 }
 
 std::string shape_key(const lym_batch *b)
@@ -489,10 +525,12 @@ int lym_batch_observables(lym_batch *b, double *out)
 
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target)
 {
-    const int order = kind & ~lym::SCHED_LDS;
-    if (!b || grid < 0 || (kind & ~(lym::SCHED_LDS | 3)) || order > lym::SCHED_COLUMN ||
+    const int order = kind & 3;
+    if (!b || grid < 0 || (kind & ~(lym::SCHED_LDS | lym::SCHED_TWO_ROW | 3)) || order > lym::SCHED_COLUMN ||
         (order == lym::SCHED_COLUMN && target < 1))
         return LYM_ERR_ARG;
+    if ((kind & lym::SCHED_TWO_ROW) && b->prec != LYM_PREC_MIXED)
+        return bfail(b, LYM_ERR_ARG, "the two-row form is for the mixed path (FP64 multiplies out all rows)");
     BHIP(b, hipSetDevice(b->device));
     BHIP(b, hipStreamSynchronize(b->stream));
     return apply_schedule(b, grid, kind, target);
@@ -527,6 +565,7 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             b->cur = cur0;
+            b->form = 3;
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
                 return rc;
@@ -538,7 +577,8 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
             b->graph_key = key;
         }
         BHIP(b, hipGraphLaunch(b->graph, b->stream));
-        if (nsteps & 1) b->cur = 1 - b->cur;
+        // nsteps steps, plus the two form conversions of the two-row flow
+        if ((nsteps + (b->rows == 2 ? 2 : 0)) & 1) b->cur = 1 - b->cur;
     } else {
         if ((rc = enqueue_flow(b, eps, rows, measure_every))) return rc;
     }
@@ -598,16 +638,39 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
         hipEvent_t e0, e1;
         BHIP(b, hipEventCreate(&e0));
         BHIP(b, hipEventCreate(&e1));
+        const bool mixed = b->prec == LYM_PREC_MIXED;
+        // The two-row candidates read a two-row copy of every lattice and write
+        // a scratch field; both live only while tuning (the field stays as it is).
+        struct Scratch {
+            float2 *p[2] = {nullptr, nullptr};
+            ~Scratch()
+            {
+                for (float2 *q : p)
+                    if (q) (void)hipFree(q);
+            }
+        } r2;
+        if (mixed) {
+            for (auto &q : r2.p) BHIP(b, hipMalloc(&q, sizeof(float2) * 6 * size_t(b->total_links)));
+            for (const ConfDesc &d : b->desc)
+                lym::k_to_rows2<float><<<dim3(nblocks(d.g.V, 256), 4), 256, 0, b->stream>>>(
+                    static_cast<const float2 *>(b->W[b->cur]) + d.link_off, r2.p[0] + d.link_off / 9 * 6, d.g.V);
+            BLAUNCHED(b, "k_to_rows2");
+        }
         auto time_stage1 = [&](float *out) -> int {
             float best_c = 1e30f;
             for (int rep = 0; rep < 4; rep++) {
                 BHIP(b, hipEventRecord(e0, b->stream));
-                if (b->prec == LYM_PREC_FP64)
+                if (b->prec == LYM_PREC_FP64) {
                     launch_stage<double, 1>(b, static_cast<double2 *>(b->W[b->cur]),
                                             static_cast<double2 *>(b->W[1 - b->cur]), static_cast<double *>(b->acc), 0.0);
-                else
+                } else if (b->rows == 2) {
+                    b->form = 2;
+                    launch_stage<float, 1>(b, r2.p[0], r2.p[1], static_cast<float *>(b->acc), 0.0);
+                    b->form = 3;
+                } else {
                     launch_stage<float, 1>(b, static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
                                            static_cast<float *>(b->acc), 0.0);
+                }
                 BLAUNCHED(b, "k_batch_flow_stage");
                 BHIP(b, hipEventRecord(e1, b->stream));
                 BHIP(b, hipEventSynchronize(e1));
@@ -619,7 +682,7 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
             return LYM_OK;
         };
         int rc;
-        for (auto cd : lym::schedule_candidates()) {
+        for (auto cd : lym::schedule_candidates(mixed)) {
             if ((rc = apply_schedule(b, 0, cd.first, cd.second))) return rc;
             float ms;
             if ((rc = time_stage1(&ms))) return rc;

@@ -44,7 +44,11 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
 // LDS as flow_block, and 4: per lattice, slab staging where the lattice fits
 // it and the run's own links elsewhere (the lattice of a tile, and so the
 // choice, is uniform over the workgroup); both modes stage into one array.
-template <typename S, int STAGE, bool REUNIT, int LDS>
+// ROWS = 2: the fields are in two-row form, 6 complexes per link, so a
+// lattice's field starts at link_off / 9 * 6.
+template <int ROWS> LYM_DI long field_off(const ConfDesc &d) { return ROWS == 2 ? d.link_off / 9 * 6 : d.link_off; }
+
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
 __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
                                                           S *__restrict__ acc, const ConfDesc *__restrict__ desc,
                                                           int nconf, long ntiles, Sched sc, double eps)
@@ -56,23 +60,20 @@ __global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restric
         if (tile < 0) continue;
         const int c = find_conf(desc, nconf, tile);
         const ConfDesc d = desc[c];
-        const long site0 = (tile - d.tile0) * kSitesPerBlock;
+        const long site0 = (tile - d.tile0) * kSitesPerBlock, off = field_off<ROWS>(d);
         if constexpr (LDS == 4) {
             if (d.slab)
-                flow_block<S, STAGE, REUNIT, 3>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
-                                                lds);
+                flow_block<S, STAGE, REUNIT, 3, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
             else
-                flow_block<S, STAGE, REUNIT, 1>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
-                                                lds);
+                flow_block<S, STAGE, REUNIT, 1, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
         } else {
-            flow_block<S, STAGE, REUNIT, LDS>(win + d.link_off, wout + d.link_off, acc + d.acc_off, d.g, site0, eps,
-                                              lds);
+            flow_block<S, STAGE, REUNIT, LDS, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
         }
     }
 }
 
 // observables: one partial per tile (192 threads = 64 sites x 3 plane pairs)
-template <typename S>
+template <typename S, int ROWS = 3>
 __global__ __launch_bounds__(192) void k_batch_observables(const F2<S> *__restrict__ f, const ConfDesc *__restrict__ desc,
                                                            int nconf, long ntiles, ksum *__restrict__ partials)
 {
@@ -84,13 +85,13 @@ __global__ __launch_bounds__(192) void k_batch_observables(const F2<S> *__restri
         ksum v[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
         if (s < d.g.V) {
             double o[3];
-            const F2<S> *fc = f + d.link_off;
+            const F2<S> *fc = f + field_off<ROWS>(d);
             if (pair == 0)
-                pair_observables<S, S, 0>(fc, d.g, s, o);
+                pair_observables<S, S, 0, ROWS>(fc, d.g, s, o);
             else if (pair == 1)
-                pair_observables<S, S, 1>(fc, d.g, s, o);
+                pair_observables<S, S, 1, ROWS>(fc, d.g, s, o);
             else
-                pair_observables<S, S, 2>(fc, d.g, s, o);
+                pair_observables<S, S, 2, ROWS>(fc, d.g, s, o);
             v[0].s = o[0];
             v[1].s = o[1];
             v[2].s = o[2];

@@ -360,27 +360,38 @@ LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, unsigned s)
     return m;
 }
 
+// FP32 links (the mixed path) hold row 2 = conj(row0 x row1) formed in FP32
+// from their stored rows 0, 1: store_link writes it so whatever row 2 of m
+// is, and the upload kernel derives it too.  The two-row form (below) is then
+// a lossless compression of an FP32 field: a reader sees the same values
+// whichever form the field is in, so the choice of form changes no result.
+// FP64 links are stored as they are (the FP64 path multiplies out all rows).
 template <typename S, typename R, int AUX = 0>
 LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, unsigned s, const su3<R> &m)
 {
     using F2 = vec2_t<S>;
     const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 9 * V * long(sizeof(F2)));
     const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+    su3<S> t;
+#pragma unroll
+    for (int k = 0; k < 9; k++) t.e[k] = cplx<S>{S(m.e[k].re), S(m.e[k].im)};
+    if constexpr (sizeof(S) == 4) complete_row2(t);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
         F2 v;
-        v.x = S(m.e[k].re);
-        v.y = S(m.e[k].im);
+        v.x = t.e[k].re;
+        v.y = t.e[k].im;
         buf_store<F2, AUX>(v, r, off, unsigned(k) * es);
     }
 }
 
 // Links in two-row form (the mixed flow, chosen per lattice shape): a
 // direction plane holds the first two rows only, plane[k*V + s] for
-// k = 0..5, and row 2 = conj(row0 x row1) is rebuilt on load, in the
-// precision R (the links are SU(3): the mixed path reunitarises every link it
-// writes).  A third fewer bytes per link read and written; where a product
-// uses rows 0, 1 of a link only, the rebuild is dead code and drops out.
+// k = 0..5, and row 2 = conj(row0 x row1) is rebuilt on load in the storage
+// precision S, exactly as store_link forms the row 2 it stores for FP32
+// links, then converted to R.  A third fewer bytes per link read and
+// written; where a product uses rows 0, 1 of a link only, the rebuild is dead
+// code and drops out.
 template <int ROWS> constexpr int plane_elems() { return ROWS == 2 ? 6 : 9; }
 
 template <typename R, typename S, int ROWS = 3>
@@ -392,14 +403,14 @@ LYM_DI su3<R> load_link_r(const vec2_t<S> *__restrict__ plane, long V, unsigned 
         using F2 = vec2_t<S>;
         const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 6 * V * long(sizeof(F2)));
         const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
-        su3<R> m;
+        su3<S> m;
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             const F2 v = buf_load<F2>(r, off, unsigned(k) * es);
-            m.e[k] = cplx<R>{R(v.x), R(v.y)};
+            m.e[k] = cplx<S>{v.x, v.y};
         }
         complete_row2(m);
-        return m;
+        return convert<R>(m);
     }
 }
 

@@ -710,29 +710,39 @@ __global__ __launch_bounds__(256) void k_to_rows2(const F2<S> *__restrict__ in, 
     for (int k = 0; k < 6; k++) out[(mu * 6 + k) * V + s] = in[(mu * 9 + k) * V + s];
 }
 
-// Two-row field -> three-row field, row 2 = conj(row0 x row1) formed in FP64.
+// Two-row field -> three-row field: row 2 is the one load_link_r rebuilds
+// and store_link stores, so the conversion loses nothing.
 template <typename S>
 __global__ __launch_bounds__(256) void k_from_rows2(const F2<S> *__restrict__ in, F2<S> *__restrict__ out, long V)
 {
     const long s = (long)blockIdx.x * 256 + threadIdx.x;
     if (s >= V) return;
     const long mu = blockIdx.y;
-    const su3<double> m = load_link_r<double, S, 2>(in + mu * 6 * V, V, unsigned(s));
-    store_link<S, double>(out + mu * 9 * V, V, unsigned(s), m);
+    const su3<S> m = load_link_r<S, S, 2>(in + mu * 6 * V, V, unsigned(s));
+    store_link<S, S>(out + mu * 9 * V, V, unsigned(s), m);
 }
 
-// reference layout (V x 18 doubles for one mu) -> SoA plane of direction mu
+// reference layout (V x 18 doubles for one mu) -> SoA plane of direction mu.
+// FP64 fields take every element as it is; FP32 fields take rows 0, 1
+// rounded and derive row 2 from them (store_link).
 template <typename S>
 __global__ void k_aos_to_soa(const double *__restrict__ aos, F2<S> *__restrict__ plane, long V)
 {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= V) return;
+    if constexpr (sizeof(S) == 4) {
+        su3<double> m;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        F2<S> v;
-        v.x = S(aos[s * 18 + 2 * k]);
-        v.y = S(aos[s * 18 + 2 * k + 1]);
-        plane[k * V + s] = v;
+        for (int k = 0; k < 9; k++) m.e[k] = cplx<double>{k < 6 ? aos[s * 18 + 2 * k] : 0.0, k < 6 ? aos[s * 18 + 2 * k + 1] : 0.0};
+        store_link<S, double>(plane, V, unsigned(s), m);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            F2<S> v;
+            v.x = S(aos[s * 18 + 2 * k]);
+            v.y = S(aos[s * 18 + 2 * k + 1]);
+            plane[k * V + s] = v;
+        }
     }
 }
 

@@ -1,8 +1,8 @@
 // Host-side launchers of the heavy kernel families: the RK stage kernels of
 // the single-lattice and batch paths and the Metropolis sub-sweep kernels.
 // Each family is instantiated for one link precision per translation unit
-// (lym_stage_*.hip: single lattice, lym_stage_f32r2.hip: its two-row form,
-// lym_bstage_*.hip: batch, lym_metro_*.hip: Metropolis; f64 and f32), so the device code of the
+// (lym_stage_*.hip: single lattice, lym_bstage_*.hip: batch, the *_f32r2
+// units their two-row forms, lym_metro_*.hip: Metropolis; f64 and f32), so the device code of the
 // library compiles in parallel; every other translation unit sees the extern
 // declarations below.
 #pragma once
@@ -54,6 +54,20 @@ void launch_batch_stage(int lds, unsigned grid, hipStream_t st, const F2<S> *win
     case 2: k_batch_flow_stage<S, STAGE, re, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
     case 3: k_batch_flow_stage<S, STAGE, re, 3><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
     default: k_batch_flow_stage<S, STAGE, re, 4><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    }
+}
+
+// The mixed batch stage over fields in two-row form (ROWS = 2; lym_bstage_f32r2.hip).
+template <int STAGE>
+void launch_batch_stage_r2(int lds, unsigned grid, hipStream_t st, const F2<float> *win, F2<float> *wout, float *acc,
+                           const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps)
+{
+    switch (lds) {
+    case 0: k_batch_flow_stage<float, STAGE, true, 0, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 1: k_batch_flow_stage<float, STAGE, true, 1, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 2: k_batch_flow_stage<float, STAGE, true, 2, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 3: k_batch_flow_stage<float, STAGE, true, 3, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    default: k_batch_flow_stage<float, STAGE, true, 4, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
     }
 }
 
@@ -121,6 +135,14 @@ extern template void launch_flow_stage_r2<2>(int, unsigned, hipStream_t, const F
                                              Sched, double);
 extern template void launch_flow_stage_r2<3>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *, Geom,
                                              Sched, double);
+#endif
+#ifndef LYM_INSTANTIATE_BSTAGES_R2
+extern template void launch_batch_stage_r2<1>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *,
+                                              const ConfDesc *, int, long, Sched, double);
+extern template void launch_batch_stage_r2<2>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *,
+                                              const ConfDesc *, int, long, Sched, double);
+extern template void launch_batch_stage_r2<3>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *,
+                                              const ConfDesc *, int, long, Sched, double);
 #endif
 #ifndef LYM_INSTANTIATE_METRO
 LYM_METRO_EXTERN(double)

@@ -287,3 +287,32 @@ def test_ensemble_member_batch_flow_against_oracle(ensemble, member):
         assert abs(got[r, 1] / want[r, 1] - 1) < 1e-12, r
         assert abs(got[r, 3] / want[r, 3] - 1) < 1e-12, r
         assert abs(got[r, 2] - want[r, 2]) < 1e-12 * scale, r
+
+
+@pytest.mark.parametrize("grid,kind,target,lds", [(0, "linear", 0, True), (0, "column", 64, True), (8, "contig", 0, False)])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_two_row_batch_matches_single_lattice_path(inputs, grid, kind, target, lds, use_graph):
+    """A mixed batch whose lattices are kept in two-row form while the flow
+    runs gives each lattice the single-lattice flow bit for bit in its links,
+    the single lattice in three-row form (FP32 links hold the row 2 that the
+    two-row form rebuilds; observables: the same terms summed in another
+    order), and the links come back in three-row form."""
+    eps, tau, every = 0.02, 0.12, 2
+    singles = []
+    for c, L in enumerate(inputs):
+        with LatticeContext(DIMS[c], "mixed") as ctx:
+            ctx.set_schedule("linear", 0, lds, two_row=False)
+            ctx.upload(L)
+            singles.append((ctx.flow(eps, tau, every), ctx.download()))
+    with BatchContext(DIMS, "mixed") as b:
+        b.set_schedule(grid, kind, target, lds, two_row=True)
+        assert b.info()["two_row"]
+        for c, L in enumerate(inputs):
+            b.upload(c, L)
+        series = b.flow(eps, tau, every, use_graph=use_graph)
+        for c in range(len(DIMS)):
+            s, links = singles[c]
+            np.testing.assert_array_equal(b.download(c), links)
+            m = ~np.isnan(s[:, 1])
+            assert np.allclose(series[c][m][:, [0, 1, 3]], s[m][:, [0, 1, 3]], rtol=1e-14, atol=0)
+            assert np.allclose(series[c][m][:, 2], s[m][:, 2], rtol=0, atol=1e-13)

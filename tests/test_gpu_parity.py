@@ -252,7 +252,10 @@ def test_lds_variant_without_whole_rows(dims):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("mixed", 1e-6)])
 def test_check_links_reports_health(precision, tol):
-    """NaN and non-unitary links are reported (count, max deviation), not fatal."""
+    """NaN and non-unitary links are reported (count, max deviation), not fatal.
+    A link scaled by 1.1 deviates by 0.331 in FP64 (det 1.331); an FP32
+    field keeps the scaled rows 0, 1 and derives row 2 from them (scaled by
+    1.21), so there it deviates by 0.4641 (det 1.4641)."""
     dims = (4, 4, 4, 4)
     links = orc.hot_start(dims, 21)
     with LatticeContext(dims, precision) as ctx:
@@ -265,7 +268,7 @@ def test_check_links_reports_health(precision, tol):
         ctx.upload(broken)
         dev, bad = ctx.check_links()
         assert bad == 1
-        assert abs(dev - 0.331) < 1e-5
+        assert abs(dev - (0.331 if precision == "fp64" else 0.4641)) < 1e-5
         ctx.set_cold()
         assert ctx.check_links() == (0.0, 0)
 
@@ -424,9 +427,10 @@ def test_staples_large_lattice_against_small_path():
 def test_two_row_form_flow(dims, kind, lds):
     """Mixed path with the links in two-row form while the flow runs (row 2
     rebuilt by unitarity where read; slab, tile, own-link and no staging):
-    one step within FP32 rounding of the oracle, and a 5-step series within
-    the mixed path's bounds of the three-row mixed path; the field comes back
-    in three-row form."""
+    one step within FP32 rounding of the oracle, and a 5-step series and the
+    final links equal to the three-row mixed path's bit for bit (FP32 links
+    hold the row 2 that the two-row form rebuilds); the field comes back in
+    three-row form."""
     links = orc.hot_start(dims, 61)
     want = orc.flow_step(links, dims, 0.02)
     target = 256 if kind == "column" else 0
@@ -440,14 +444,14 @@ def test_two_row_form_flow(dims, kind, lds):
         s2 = ctx.flow(0.02, 0.12, 1)
         dev, bad = ctx.check_links()
         assert bad == 0 and dev < 1e-6
+        l2 = ctx.download()
     with LatticeContext(dims, "mixed") as ctx:
         ctx.set_schedule(kind, target, lds)
         ctx.upload(links)
         s3 = ctx.flow(0.02, 0.12, 1)
-    scale = orc.observables(links, dims)[3]
-    assert np.array_equal(s2[:, 0], s3[:, 0])
-    assert np.all(np.abs(s2[:, [1, 3]] / s3[:, [1, 3]] - 1) < 1e-6)
-    assert np.all(np.abs(s2[:, 2] - s3[:, 2]) < 1e-6 * max(scale, 1.0))
+        l3 = ctx.download()
+    np.testing.assert_array_equal(s2, s3)
+    np.testing.assert_array_equal(l2, l3)
 
 
 def test_two_row_form_graph_replays_and_tuning():

@@ -78,18 +78,19 @@ def noalloc_observables(links, dims, precision):
 
 
 def ctx_observables(links, dims, precision):
+    """lym_observables of the uploaded links, and the links as the context
+    stores them (FP32 fields derive row 2 from the rounded rows 0, 1)"""
     with LatticeContext(dims, precision) as ctx:
         ctx.upload(links)
-        return np.array(ctx.observables())
+        return np.array(ctx.observables()), ctx.download()
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_noalloc_matches_lym_observables_twice(precision):
     big, small = (8, 8, 8, 16), (4, 4, 6, 8)
     for dims, seed in [(big, 71), (small, 72)]:
-        links = orc.hot_start(dims, seed)
-        got = noalloc_observables(links, dims, precision)
-        want = ctx_observables(links, dims, precision)
+        want, stored = ctx_observables(orc.hot_start(dims, seed), dims, precision)
+        got = noalloc_observables(stored, dims, precision)
         assert abs(got[0] / want[0] - 1) < 1e-12, (dims, got, want)
         assert abs(got[1] / want[1] - 1) < 1e-12, (dims, got, want)
         assert abs(got[2] - want[2]) < 1e-12 * want[3], (dims, got, want)

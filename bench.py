@@ -257,7 +257,8 @@ def main():
     V = int(np.prod(dims))
     full_steps = int(w["tau"] / eps) - 1  # reference semantics: int(tau/eps) - 1 steps
     steps = args.steps if args.steps else full_steps
-    tau = (steps + 1) * eps
+    # int(tau / eps) = steps + 1 rows, whatever the rounding of (steps + 1) * eps
+    tau = (steps + 1.5) * eps
     log(f"[bench] workload {args.workload} dims={dims} V={V} steps={steps} warmup={args.warmup}")
 
     # thermalise in FP64 on the GPU
@@ -287,12 +288,16 @@ def main():
             down_s = time.perf_counter() - t
             tuner = ctx.autotune(args.tune_cache)
             if args.warmup:
-                ctx.flow_steps(eps, args.warmup)
+                # the warm-up runs every kernel of the timed flow (stages, form
+                # conversions, observables, reduction) and sizes its measurement
+                # buffer (one slot per step), so the timed call allocates nothing
+                ctx.flow(eps, (args.warmup + 1.5) * eps, 1)
                 ctx.synchronize()
                 ctx.upload(links)
             series, el = timed_flow(ctx, eps, tau, every)
+            assert series.shape[0] == steps + 1
             if steps != full_steps:
-                # the error statement always covers the whole flow to t = tau - eps,
+                # the error statement always covers the whole flow to t = (full_steps) eps,
                 # whatever the timed step count (untimed)
                 ctx.upload(links)
                 series = ctx.flow(eps, w["tau"], every)
@@ -322,7 +327,7 @@ def main():
                       "note": "one upload and one download of the configuration (GluonField layout in host memory) "
                               "around the timed flow; value excludes them"},
         "config": {"workload": args.workload, "lattice": "x".join(map(str, dims)), "beta": w["beta"],
-                   "start": w.get("start", "cold"), "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "timed_flow_t_max": tau - eps,
+                   "start": w.get("start", "cold"), "metropolis_sweeps": sweeps, "seed": w["seed"], "plaquette_start": plaq0, "flow_eps": eps, "timed_flow_t_max": steps * eps,
                    "measure_every": every, "precision": main_prec},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,

@@ -147,17 +147,21 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
  * slices stay in its L2).  Every schedule and staging mode computes each link
  * the same way, so the links agree bit for bit across them (the observables
  * are the same site terms summed in another order), the two-row form
- * included: it drops the row 2 that an FP32 field derives anyway.  Default: COLUMN | LDS with 8192 sites; the fastest choice
- * depends on shape and precision, which lym_autotune measures. */
+ * included: it drops the row 2 that an FP32 field derives anyway.  Default:
+ * COLUMN | LDS with 4096 sites, with TWO_ROW in LYM_PREC_MIXED; the fastest
+ * choice depends on shape and precision, which lym_autotune measures. */
 #define LYM_SCHED_LINEAR 0
 #define LYM_SCHED_CONTIG 1
 #define LYM_SCHED_COLUMN 2
 #define LYM_SCHED_LDS 16 /* or-ed into a kind: stage each 64-site run's own links in LDS */
-/* or-ed into a kind, LYM_PREC_MIXED only: while a flow call runs (lym_flow,
- * lym_flow_steps, lym_profile_flow) the links are kept as their first two
- * rows, row 2 = conj(row0 x row1) rebuilt where a kernel reads a link: a
- * third fewer bytes per link, more arithmetic; the call converts the field at
- * entry and back before it returns.  lym_autotune tries it per shape. */
+/* or-ed into a kind, LYM_PREC_MIXED only: the flow calls (lym_flow,
+ * lym_flow_steps, lym_profile_flow) keep the links as their first two rows,
+ * row 2 = conj(row0 x row1) rebuilt where a kernel reads a link: a third
+ * fewer bytes per link, more arithmetic.  A flow call converts the field on
+ * entry and leaves it in two-row form, which lym_observables reads as it is;
+ * the calls that read links as three rows (download, staples, derivative,
+ * Metropolis, link check, autotune) convert it back first.  lym_autotune
+ * tries it per shape. */
 #define LYM_SCHED_TWO_ROW 32
 int lym_set_schedule(lym_ctx *ctx, int kind, int target);
 /* Times every candidate schedule on the context's own field (RK stage 1,

@@ -311,7 +311,9 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(ConfDesc) * nconf, hipMemcpyHostToDevice) != hipSuccess)
         return bail(LYM_ERR_HIP);
     if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
-    if (apply_schedule(b, 0, lym::SCHED_COLUMN | lym::SCHED_LDS, 8192) != LYM_OK)
+    // as the single-lattice default: the mixed flow in two-row form
+    const int kind = lym::SCHED_COLUMN | lym::SCHED_LDS | (precision == LYM_PREC_MIXED ? lym::SCHED_TWO_ROW : 0);
+    if (apply_schedule(b, 0, kind, 4096) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
     // cold start
     for (int c = 0; c < nconf; c++) {

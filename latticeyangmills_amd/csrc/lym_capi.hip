@@ -202,8 +202,13 @@ void launch_stage(const lym_ctx *c, const lym::F2<S> *win, lym::F2<S> *wout, S *
 }
 
 // Puts W[cur] into form `rows` (3: three-row, 2: two-row, the mixed flow's
-// SCHED_TWO_ROW) by a conversion into W[1 - cur]; the flow entry points take
-// the field into the schedule's form and back to three rows before returning.
+// SCHED_TWO_ROW) by a conversion into W[1 - cur].  The flow entry points take
+// the field into the schedule's form and leave it there, so a caller that
+// alternates one flow step and one measurement (the reference's loop) pays
+// no conversion; lym_observables reads either form, and the entry points
+// that read links as three rows (download, staples, force, Metropolis, link
+// check, autotune) convert back first (readable).  Two-row form implies
+// lexicographic order: Metropolis leaves even-odd order in three-row form.
 int to_form(lym_ctx *c, int rows)
 {
     if (c->form == rows) return LYM_OK;
@@ -218,6 +223,13 @@ int to_form(lym_ctx *c, int rows)
     c->cur ^= 1;
     c->form = rows;
     return LYM_OK;
+}
+
+// the links as three rows in lexicographic order
+int readable(lym_ctx *c)
+{
+    if (int rc = to_form(c, 3)) return rc;
+    return to_lex(c);
 }
 
 template <typename S> int enqueue_step(lym_ctx *c, double eps, hipEvent_t *ev)
@@ -343,7 +355,9 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
     if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->check, 2 * sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
-    if (apply_schedule(c, lym::SCHED_COLUMN | lym::SCHED_LDS, 8192) != LYM_OK)
+    // the schedule the autotuner picks at 32^3x64 (the mixed flow in two-row form)
+    const int kind = lym::SCHED_COLUMN | lym::SCHED_LDS | (precision == LYM_PREC_MIXED ? lym::SCHED_TWO_ROW : 0);
+    if (apply_schedule(c, kind, 4096) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
     int rc = lym_set_cold(c);
     if (rc != LYM_OK) return bail(rc);
@@ -403,6 +417,7 @@ int lym_upload_links(lym_ctx *c, const double *const by_mu[4])
         LYM_LAUNCHED(c, "k_aos_to_soa");
     }
     c->eo = false;
+    c->form = 3;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -413,7 +428,7 @@ int lym_download_links(lym_ctx *c, double *const by_mu[4])
     for (int mu = 0; mu < 4; mu++)
         if (!by_mu[mu]) return fail(c, LYM_ERR_ARG, "null link array");
     LYM_HIP(c, hipSetDevice(c->device));
-    if (int rc = to_lex(c)) return rc;
+    if (int rc = readable(c)) return rc;
     const long V = c->g.V;
     for (int mu = 0; mu < 4; mu++) {
         if (c->prec == LYM_PREC_FP64)
@@ -439,6 +454,7 @@ int lym_set_cold(lym_ctx *c)
         lym::k_cold<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V);
     LYM_LAUNCHED(c, "k_cold");
     c->eo = false;
+    c->form = 3;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -454,6 +470,7 @@ int lym_set_hot(lym_ctx *c, uint64_t seed)
         lym::k_hot<float><<<blocks_for(4 * V, 256), 256, 0, c->stream>>>(static_cast<float2 *>(c->W[c->cur]), V, seed);
     LYM_LAUNCHED(c, "k_hot");
     c->eo = false;
+    c->form = 3;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }
@@ -462,7 +479,7 @@ static int export_force(lym_ctx *c, int mu, int what, double *out)
 {
     if (!c || !out || mu < 0 || mu > 3) return c ? fail(c, LYM_ERR_ARG, "bad mu or null output") : LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
-    if (int rc = to_lex(c)) return rc;
+    if (int rc = readable(c)) return rc;
     const long V = c->g.V;
     if (c->prec == LYM_PREC_FP64)
         lym::k_export_force<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
@@ -503,7 +520,7 @@ int lym_flow_steps(lym_ctx *c, double eps, int nsteps)
         int rc = step(c, eps);
         if (rc) return rc;
     }
-    return to_form(c, 3);
+    return LYM_OK;
 }
 
 int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *series, int max_rows, int *rows_out)
@@ -524,12 +541,14 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             r = step(c, eps);
             if (!r && t % measure_every == 0) r = observables(c, slot++);
         }
-        return r ? r : to_form(c, 3);
+        return r;
     };
     if (c->use_graph) {
-        // key: the flow parameters (the buffers are fixed while the graph lives)
+        // key: the flow parameters and the form the field starts in (the
+        // buffers are fixed while the graph lives; a schedule change drops it)
         char key[96];
-        snprintf(key, sizeof key, "%a|%d|%d", eps, rows, measure_every);
+        const int form0 = c->form;
+        snprintf(key, sizeof key, "%a|%d|%d|%d", eps, rows, measure_every, form0);
         const int k = c->cur;
         if (!c->graph[k] || c->graph_key[k] != key) {
             if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
@@ -540,7 +559,7 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(c->stream, &g);
             c->cur = k;  // capture enqueued nothing: the field is where it was
-            c->form = 3;
+            c->form = form0;
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
                 return rc;
@@ -552,8 +571,10 @@ int lym_flow(lym_ctx *c, double eps, double tau, int measure_every, double *seri
             c->graph_key[k] = key;
         }
         LYM_HIP(c, hipGraphLaunch(c->graph[k], c->stream));
-        // rows - 1 steps, plus the two form conversions of the two-row flow
-        if (((rows - 1) + (c->rows == 2 ? 2 : 0)) & 1) c->cur = 1 - k;
+        // rows - 1 steps, plus the form conversion at the start if the field
+        // was not in the schedule's form
+        if (((rows - 1) + (form0 != c->rows ? 1 : 0)) & 1) c->cur = 1 - k;
+        c->form = c->rows;
     } else if ((rc = enqueue())) {
         return rc;
     }
@@ -628,8 +649,10 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
     LYM_LAUNCHED(c, "k_zero_u64");
     // The sweeps run on the links in even-odd order, which they are left in
     // (to_lex reorders them when another entry point reads them).
-    if (nsweeps > 0)
+    if (nsweeps > 0) {
+        if (int rc = to_form(c, 3)) return rc;
         if (int rc = to_order(c, true)) return rc;
+    }
     if (c->metro_target > 0 && !c->d_morder) {
         const lym::HostSchedule h =
             lym::metro_column_schedule({c->dims[0], c->dims[1], c->dims[2], c->dims[3]}, c->metro_target);
@@ -661,7 +684,10 @@ int lym_metropolis(lym_ctx *c, double beta, double eps, int nhit, int nsweeps, i
 int lym_check_links(lym_ctx *c, double *max_dev, int64_t *nonfinite)
 {
     if (!c) return LYM_ERR_ARG;
-    // a maximum and a count over all links: either site order will do
+    LYM_HIP(c, hipSetDevice(c->device));
+    // a maximum and a count over all links of the three-row field: either
+    // site order will do
+    if (int rc = to_form(c, 3)) return rc;
     unsigned long long *d = c->check;
     lym::k_zero_u64<<<1, 64, 0, c->stream>>>(d, 2);
     LYM_LAUNCHED(c, "k_zero_u64");
@@ -710,7 +736,6 @@ int lym_profile_flow(lym_ctx *c, double eps, int nsteps, int measure_every, doub
         }
     }
     LYM_HIP(c, hipEventRecord(ev[1], c->stream));
-    if ((rc = to_form(c, 3))) return rc;
     LYM_HIP(c, hipEventSynchronize(ev[1]));
     float ms = 0;
     LYM_HIP(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -753,7 +778,7 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
 {
     if (!c) return LYM_ERR_ARG;
     LYM_HIP(c, hipSetDevice(c->device));
-    if (int rc = to_lex(c)) return rc;
+    if (int rc = readable(c)) return rc;
     std::ostringstream key;
     key << "gfx950|single|prec" << c->prec << "|" << c->dims[0] << "x" << c->dims[1] << "x" << c->dims[2] << "x"
         << c->dims[3];

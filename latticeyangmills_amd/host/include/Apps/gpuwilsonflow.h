@@ -40,14 +40,26 @@ public:
         m_batchBytes = batchBytes;
     }
 
+    /// on (default): before its first flow the App picks the flow schedule for
+    /// the lattice shape by timing the candidates (lym_autotune /
+    /// lym_batch_autotune; no result changes), keeping the choice in
+    /// cachePath when one is given; off: the library's default schedule
+    void setAutotune(bool on, const std::string &cachePath = "")
+    {
+        m_autotune = on;
+        m_tuneCache = cachePath;
+    }
+
 private:
     void computeObservables();
+    void tune(DeviceField &d);
     void flowBatch(size_t first, size_t last);
     /// output of one flowed configuration: scales, observables, flow/conf%04d.dat
     void finishConfiguration(int confNum, const std::vector<std::array<double, 4>> &series);
     double m_tauFinal, m_epsilon;
     int m_precision, m_measureEvery;
-    bool m_batchFlow = true;
+    bool m_batchFlow = true, m_autotune = true;
+    std::string m_tuneCache;
     double m_batchBytes = 32.0 * (1ull << 30);
     std::string m_inputDir = ".", m_outputDir = ".";
     std::vector<std::string> m_inputConfList;

@@ -52,6 +52,12 @@ public:
     };
     Health health();
 
+    // Picks the flow schedule for this lattice shape and precision once, by
+    // timing the candidates on the current device field (lym_autotune); the
+    // choice changes no result.  cachePath (may be empty) keeps the choice
+    // across runs.  Returns the tuner's summary (empty after the first call).
+    std::string autotune(const std::string &cachePath);
+
     // throws std::runtime_error with lym_last_error() on failure
     void check(int rc, const char *what);
 
@@ -60,5 +66,5 @@ private:
     GluonField *m_host;
     lym_ctx *m_ctx = nullptr;
     int m_precision;
-    bool m_hostNewer = true, m_deviceNewer = false;
+    bool m_hostNewer = true, m_deviceNewer = false, m_tuned = false;
 };

@@ -69,6 +69,17 @@ void DeviceField::toHost()
     m_deviceNewer = false;
 }
 
+std::string DeviceField::autotune(const std::string &cachePath)
+{
+    if (m_tuned) return "";
+    toDevice();
+    char summary[256] = {0};
+    check(lym_autotune(m_ctx, cachePath.empty() ? nullptr : cachePath.c_str(), summary, int(sizeof summary)),
+          "lym_autotune");
+    m_tuned = true;
+    return summary;
+}
+
 DeviceField::Health DeviceField::health()
 {
     toDevice();

@@ -140,6 +140,12 @@ void GpuWilsonFlow::flowBatch(size_t first, size_t last)
         for (int mu = 0; mu < 4; mu++) p[mu] = (*m_lat)[mu].lattice.data()->mat.data();
         check(lym_batch_upload_links(b, c, p), "lym_batch_upload_links");
     }
+    if (m_autotune) {
+        char summary[256] = {0};
+        check(lym_batch_autotune(b, m_tuneCache.empty() ? nullptr : m_tuneCache.c_str(), summary, int(sizeof summary)),
+              "lym_batch_autotune");
+        if (first == 0) printf("batch flow schedule: %s\n", summary);
+    }
     const int rows = std::max(1, int(m_tauFinal / m_epsilon));
     std::vector<std::array<double, 4>> all(size_t(n) * size_t(rows));
     int got = 0;
@@ -155,10 +161,18 @@ void GpuWilsonFlow::flowBatch(size_t first, size_t last)
     }
 }
 
+void GpuWilsonFlow::tune(DeviceField &d)
+{
+    if (!m_autotune) return;
+    const std::string s = d.autotune(m_tuneCache);
+    if (!s.empty()) printf("flow schedule: %s\n", s.c_str());
+}
+
 std::vector<std::array<double, 4>> GpuWilsonFlow::applyWilsonFlow(int confNum)
 {
     DeviceField &d = DeviceField::of(m_lat, m_precision);
     d.toDevice();
+    tune(d);
     const int rows = int(m_tauFinal / m_epsilon);
     std::vector<std::array<double, 4>> series(size_t(std::max(rows, 1)));
     int got = 0;
@@ -206,6 +220,7 @@ void GpuWilsonFlow::flowStep()
 {
     DeviceField &d = DeviceField::of(m_lat, m_precision);
     d.toDevice();
+    tune(d);
     d.check(lym_flow_steps(d.ctx(), m_epsilon, 1), "lym_flow_steps");
     d.deviceModified();
 }

@@ -9,11 +9,14 @@
 //                              PGFG, LatticeSize, Action puregauge, Observables,
 //                              InputDir, OutputDir; optional build keys
 //                              "precision", "measureEvery", "batchFlow" (FLOW:
-//                              false flows the inputs one by one), "seed",
+//                              false flows the inputs one by one), "autotune"
+//                              (FLOW: false keeps the default schedule),
+//                              "tuneCache" (FLOW: file of tuned schedules), "seed",
 //                              "nHit", "candidate" ("exph" | "su2") and
 //                              "referenceMetropolis" (PGFG: true takes the
 //                              reference's 30 hits of R*S*T candidates))
-//   flow also takes --no-batch, generate --candidate exph|su2
+//   flow also takes --no-batch, --no-autotune, --tune-cache FILE; generate
+//   --candidate exph|su2
 //   lym_app io-selftest DIR   (host only: writes and reads back a .bin file)
 //   lym_app scales FILE VOLUME [BETA]
 //                             (host only: t0/a^2 and w0/a of a flow series file
@@ -37,9 +40,9 @@ struct Args {
     std::array<int, 4> lat{{0, 0, 0, 0}};
     double eps = 0.01, tau = 1.0, beta = 6.0;
     int therm = 0, steps = 0, confs = 1, nhit = 10, every = 1, precision = LYM_PREC_FP64, candidate = LYM_CAND_EXPH;
-    bool batch = true;
+    bool batch = true, autotune = true;
     unsigned long long seed = 0;
-    std::string input = ".", output = ".", start = "Cold";
+    std::string input = ".", output = ".", start = "Cold", tune_cache;
 };
 
 int usage()
@@ -66,6 +69,10 @@ bool parse(int argc, char **argv, Args &a)
             a.batch = false;
             continue;
         }
+        if (k == "--no-autotune") {
+            a.autotune = false;
+            continue;
+        }
         if (!(v = next())) return false;
         if (k == "--eps") a.eps = atof(v);
         else if (k == "--candidate") a.candidate = (std::string(v) == "su2") ? LYM_CAND_SU2 : LYM_CAND_EXPH;
@@ -79,6 +86,7 @@ bool parse(int argc, char **argv, Args &a)
         else if (k == "--seed") a.seed = strtoull(v, nullptr, 10);
         else if (k == "--input") a.input = v;
         else if (k == "--output") a.output = v;
+        else if (k == "--tune-cache") a.tune_cache = v;
         else if (k == "--start") a.start = v;
         else if (k == "--precision") a.precision = (std::string(v) == "mixed") ? LYM_PREC_MIXED : LYM_PREC_FP64;
         else return false;
@@ -116,6 +124,7 @@ int run_json(const char *path)
         GpuWilsonFlow app(p["tauFinal"].number(), p["epsilon"].number(), precision,
                           j.has("measureEvery") ? j["measureEvery"].integer() : 1);
         if (j.has("batchFlow")) app.setBatchFlow(j["batchFlow"].b);
+        app.setAutotune(!j.has("autotune") || j["autotune"].b, j.has("tuneCache") ? j["tuneCache"].string() : "");
         app.setAction(&act);
         if (superobs) app.addObservable(&obs);
         app.createLattice(lat);
@@ -198,6 +207,7 @@ int main(int argc, char **argv)
         if (cmd == "flow") {
             GpuWilsonFlow app(a.tau, a.eps, a.precision, a.every);
             app.setBatchFlow(a.batch);
+            app.setAutotune(a.autotune, a.tune_cache);
             GpuSuperObs obs(a.precision);
             GpuPureGauge act(a.beta, a.precision);
             app.setAction(&act);

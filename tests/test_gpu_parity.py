@@ -477,3 +477,40 @@ def test_two_row_form_graph_replays_and_tuning():
         b.set_graph(False)
         b.upload(links)
         np.testing.assert_array_equal(a.download(), b.download())
+
+
+def test_two_row_form_between_calls_is_invisible():
+    """The mixed flow leaves the links in two-row form between calls; every
+    entry point that reads them gives what it gives on the same links passed
+    through host memory (three-row form), and the reference's loop of one
+    step and one measurement per call gives lym_flow's series bit for bit."""
+    dims = (8, 4, 4, 8)
+    links = orc.hot_start(dims, 57)
+    eps = 0.02
+    readers = [
+        lambda c: np.array(c.observables()),
+        lambda c: c.staples(1),
+        lambda c: c.derivative(2),
+        lambda c: np.array(c.check_links()),
+        lambda c: c.download(),
+        lambda c: (c.metropolis(6.0, 0.24, 4, 1, 0, "exph", 5), c.download())[1],
+    ]
+    with LatticeContext(dims, "mixed") as a, LatticeContext(dims, "mixed") as b:
+        for c in (a, b):
+            c.set_schedule("column", 256, True, two_row=True)
+            c.upload(links)
+        for i, read in enumerate(readers):
+            a.flow_steps(eps, 2)
+            b.flow_steps(eps, 2)
+            b.upload(b.download())  # the same links, now in three-row form
+            np.testing.assert_array_equal(read(a), read(b), err_msg=str(i))
+        a.upload(links)
+        series = a.flow(eps, 0.105, 1)
+        a.upload(links)
+        loop = [a.observables()]
+        for _ in range(4):
+            a.flow_steps(eps, 1)
+            loop.append(a.observables())
+        np.testing.assert_array_equal(series[:, 1], [o[0] for o in loop])
+        np.testing.assert_array_equal(series[:, 3], [o[1] for o in loop])
+        np.testing.assert_array_equal(series[:, 2], [o[2] for o in loop])

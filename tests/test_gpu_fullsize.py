@@ -165,6 +165,7 @@ def test_32x32x32x64_mixed_within_stated_error():
         smx = ctx.flow(0.02, 6.0, 10)
         u = unitarity(ctx.download())
     rows = range(0, 300, 10)
+    assert abs(s64[0, 1] - 0.6136) < 5e-4  # the published equilibrium plaquette at beta 6.2
     assert max(rel(smx[r, 3], s64[r, 3]) for r in rows) < 1e-6
     assert max(abs(smx[r, 2] - s64[r, 2]) for r in rows) < 1e-4
     assert max(rel(smx[r, 1], s64[r, 1]) for r in rows) < 1e-6
@@ -215,3 +216,28 @@ def test_48x48x48x96_mixed_within_stated_error():
         assert bad == 0 and dev < 1e-6, form
     m = s64[::5]
     assert np.all(np.diff(m[:, 1]) > 0)  # the plaquette rises along the flow
+
+
+@pytest.mark.parametrize("candidate,eps,start,therm", [("exph", 0.24, "hot", 300), ("su2", 0.12, "cold", 1000)])
+def test_metropolis_equilibrium_plaquette(candidate, eps, start, therm):
+    """Production Metropolis agrees with the Wilson action statistically: on
+    16^4 at beta = 6.0, after `therm` sweeps of 10 hits, the plaquette
+    averaged over 150 more sweeps (one call per sweep, as MCUpdate) is the
+    published equilibrium value 0.5937 to 1e-3, for both candidates.  The
+    R*S*T candidate (three SU(2) rotations by eps each, at half the step;
+    18 % acceptance) drifts up from a Haar start over thousands of sweeps, so
+    it starts cold and comes down; the acceptance stays between 0.05 and 0.9."""
+    dims = (16, 16, 16, 16)
+    V = orc.volume(dims)
+    with LatticeContext(dims) as ctx:
+        if start == "hot":
+            ctx.set_hot(5)
+        acc = ctx.metropolis(6.0, eps, 10, therm, 0, candidate, 99) / (therm * 4 * V * 10)
+        assert 0.05 < acc < 0.9
+        plaq = []
+        for sw in range(150):
+            ctx.metropolis(6.0, eps, 10, 1, therm + sw, candidate, 99)
+            plaq.append(ctx.observables()[0])
+    mean = float(np.mean(plaq))
+    print(f"beta 6.0, {candidate} eps {eps}: acceptance {acc:.3f}, <P> = {mean:.6f} (spread {np.std(plaq):.1e})")
+    assert abs(mean - 0.5937) < 1e-3

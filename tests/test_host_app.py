@@ -205,3 +205,35 @@ def test_scales_of_a_flow_series(tmp_path):
     assert abs(float(out["w0/a"]) - math.sqrt(0.3 / c)) < 1e-9
     assert abs(float(out["a"].split()[0]) - scale.lattice_spacing_fm(6.2)) < 1e-12
     assert abs(float(out["t0"].split()[0]) - 0.3 / c * scale.lattice_spacing_fm(6.2) ** 2) < 1e-12
+
+
+def write_bin(path, links):
+    """(4, V, 18) links -> .bin file (per site, x fastest, the four links)"""
+    np.ascontiguousarray(links.transpose(1, 0, 2)).astype(np.float64).tofile(path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flow_resumes_bit_identically_from_a_checkpoint(tmp_path, precision):
+    """A flow stopped after 8 steps, its links written as a .bin checkpoint,
+    and resumed from that file by the drop-in App (lym_app flow, which reads
+    it with InputConf's format) continues bit for bit: the resumed series'
+    observables equal rows 8..20 of the uninterrupted flow.  In mixed
+    precision the checkpoint holds the FP32 links, whose row 2 the upload
+    derives again exactly as the device had it."""
+    from latticeyangmills_amd import LatticeContext
+    dims, eps = (4, 4, 4, 8), 0.02
+    links = orc.thermalised(dims, beta=6.0, sweeps=4, seed=13)
+    with LatticeContext(dims, precision) as ctx:
+        ctx.upload(links)
+        whole = ctx.flow(eps, 21.5 * eps, 1)  # 21 rows, t = 0 .. 20 eps
+        ctx.upload(links)
+        ctx.flow_steps(eps, 8)
+        ckpt = ctx.download()
+    (tmp_path / "ckpt").mkdir()
+    write_bin(tmp_path / "ckpt" / "conf0000.bin", ckpt)
+    run("flow", "--lattice", *dims, "--eps", eps, "--tau", 13.5 * eps, "--input", tmp_path / "ckpt", "--output",
+        tmp_path / "out", "--precision", precision, "--no-batch", "--no-autotune")
+    resumed = read_series(tmp_path / "out" / "flow" / "conf0000.dat")
+    assert resumed.shape == (13, 4)
+    np.testing.assert_array_equal(resumed[:, 1:], whole[8:21, 1:])

@@ -46,8 +46,9 @@ extern "C" {
 typedef struct lym_ctx lym_ctx;
 
 /* Creates a context for one Lx*Ly*Lz*Lt periodic lattice on GPU `device`;
- * links start cold (all 1).  Extents >= 2; one direction of the field must
- * stay below 2 GiB (V < 14.9M sites in FP64, 29.8M in mixed precision).  Replaces WilsonFlow::createLattice /
+ * links start cold (all 1).  Extents >= 2 and V < 44.7M sites in either
+ * precision (64^4 and 64^3 x 128 fit; the limit keeps every buffer offset a
+ * 31-bit integer).  Replaces WilsonFlow::createLattice /
  * App::createLattice (src/Apps/wilsonflow.cpp:87-91, src/Apps/app.cpp:41-44). */
 int lym_create(lym_ctx **ctx, const int dims[4], int precision, int device);
 int lym_destroy(lym_ctx *ctx);

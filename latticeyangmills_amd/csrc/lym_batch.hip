@@ -272,7 +272,7 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
             st *= L;
             V *= L;
         }
-        if (4 * V >= (1L << 31) || 9 * V * (precision == LYM_PREC_FP64 ? 16L : 8L) >= (1L << 31)) {
+        if (4 * V >= (1L << 31) || 48 * V >= (1L << 31)) {  // as lym_create
             delete b;
             return LYM_ERR_ARG;
         }

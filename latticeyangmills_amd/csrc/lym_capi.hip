@@ -322,9 +322,9 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
         if (dims[d] < 2) return LYM_ERR_ARG;
         V *= dims[d];
     }
-    if (4 * V >= (1L << 31)) return LYM_ERR_ARG;  // link ids are 32-bit Philox counters
-    const long cbytes = precision == LYM_PREC_FP64 ? 16 : 8;
-    if (9 * V * cbytes >= (1L << 31)) return LYM_ERR_ARG;  // one direction plane per buffer descriptor
+    // 48 V bytes per buffer descriptor (FP64: a row, FP32: two rows); it also
+    // keeps the 32-bit Philox link ids (4 V) in range
+    if (48 * V >= (1L << 31)) return LYM_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LYM_ERR_NODEV;
     if (device < 0 || device >= ndev) return LYM_ERR_ARG;

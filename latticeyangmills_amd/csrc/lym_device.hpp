@@ -304,18 +304,25 @@ template <typename To, typename From> LYM_DI su3<To> convert(const su3<From> &a)
 
 // ------------------------------------------------------------ HBM access ---
 
-// Link and accumulator accesses go through buffer resources: one descriptor
-// per direction plane (scalar registers), the matrix-element offset k*V*size
-// as the scalar soffset and the site's byte offset as the one vector offset
-// shared by all elements.  No per-element address arithmetic runs in the
-// vector ALU, and an access outside the plane reads 0 instead of faulting.
-// Requires 9 * V * sizeof(complex) < 2^31 (checked when a lattice is created).
+// Link and accumulator accesses go through buffer resources (scalar
+// registers): the element planes of a direction are taken in groups of
+// kPlanesPerRsrc<S> (6 in FP32: rows 0 and 1 of a link; 3 in FP64: one row),
+// one descriptor per group; the element's offset in its group k*V*size is the
+// scalar soffset and the site's byte offset the one vector offset shared by
+// all elements.  No per-element address arithmetic runs in the vector ALU,
+// and an access outside the group reads 0 instead of faulting.  Requires
+// 48 * V < 2^31 bytes, i.e. V < 44.7 M sites (checked when a lattice is
+// created), so that every size and offset is a non-negative 32-bit integer.
+// The two-row FP32 flow (the mixed default) reads and writes a link through
+// one descriptor; FP64 takes three, which also leaves its observables and
+// stage kernels fewer live registers than one descriptor per direction did.
 constexpr int kBufferDword3 = 0x00020000;  // raw buffer, 32-bit data format (gfx9 family)
 
 template <typename T> LYM_DI __amdgpu_buffer_rsrc_t make_rsrc(const T *base, long bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), (short)0, int(bytes), kBufferDword3);
 }
+template <typename S> constexpr int kPlanesPerRsrc = sizeof(S) == 4 ? 6 : 3;
 
 // AUX is the cache policy of the access: 0, or kStreamAux (the non-temporal
 // bit) for data that this kernel does not read again: the links a stage
@@ -344,19 +351,46 @@ template <typename T, int AUX = 0> LYM_DI void buf_store(T v, __amdgpu_buffer_rs
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, int(voff), int(soff), AUX);
 }
 
+// Elements 0 .. NE-1 of the link of site s: plane[k*V + s], the planes taken
+// in groups of kPlanesPerRsrc<S> per descriptor.
+template <typename S, int NE, int AUX = 0>
+LYM_DI void load_elems(const vec2_t<S> *__restrict__ plane, long V, unsigned s, vec2_t<S> out[NE])
+{
+    using F2 = vec2_t<S>;
+    constexpr int G = kPlanesPerRsrc<S>;
+    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+#pragma unroll
+    for (int k0 = 0; k0 < NE; k0 += G) {
+        const int n = NE - k0 < G ? NE - k0 : G;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane + long(k0) * V, long(n) * V * long(sizeof(F2)));
+#pragma unroll
+        for (int k = 0; k < n; k++) out[k0 + k] = buf_load<F2, AUX>(r, off, unsigned(k) * es);
+    }
+}
+template <typename S, int NE, int AUX = 0>
+LYM_DI void store_elems(vec2_t<S> *__restrict__ plane, long V, unsigned s, const vec2_t<S> in[NE])
+{
+    using F2 = vec2_t<S>;
+    constexpr int G = kPlanesPerRsrc<S>;
+    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+#pragma unroll
+    for (int k0 = 0; k0 < NE; k0 += G) {
+        const int n = NE - k0 < G ? NE - k0 : G;
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane + long(k0) * V, long(n) * V * long(sizeof(F2)));
+#pragma unroll
+        for (int k = 0; k < n; k++) buf_store<F2, AUX>(in[k0 + k], r, off, unsigned(k) * es);
+    }
+}
+
 // One link of a direction plane: 9 complexes at plane[k*V + s], k = 0..8.
 template <typename R, typename S>
 LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, unsigned s)
 {
-    using F2 = vec2_t<S>;
-    const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 9 * V * long(sizeof(F2)));
-    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+    vec2_t<S> v[9];
+    load_elems<S, 9>(plane, V, s, v);
     su3<R> m;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const F2 v = buf_load<F2>(r, off, unsigned(k) * es);
-        m.e[k] = cplx<R>{R(v.x), R(v.y)};
-    }
+    for (int k = 0; k < 9; k++) m.e[k] = cplx<R>{R(v[k].x), R(v[k].y)};
     return m;
 }
 
@@ -369,20 +403,17 @@ LYM_DI su3<R> load_link(const vec2_t<S> *__restrict__ plane, long V, unsigned s)
 template <typename S, typename R, int AUX = 0>
 LYM_DI void store_link(vec2_t<S> *__restrict__ plane, long V, unsigned s, const su3<R> &m)
 {
-    using F2 = vec2_t<S>;
-    const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 9 * V * long(sizeof(F2)));
-    const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
     su3<S> t;
 #pragma unroll
     for (int k = 0; k < 9; k++) t.e[k] = cplx<S>{S(m.e[k].re), S(m.e[k].im)};
     if constexpr (sizeof(S) == 4) complete_row2(t);
+    vec2_t<S> v[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        F2 v;
-        v.x = t.e[k].re;
-        v.y = t.e[k].im;
-        buf_store<F2, AUX>(v, r, off, unsigned(k) * es);
+        v[k].x = t.e[k].re;
+        v[k].y = t.e[k].im;
     }
+    store_elems<S, 9, AUX>(plane, V, s, v);
 }
 
 // Links in two-row form (the mixed flow, chosen per lattice shape): a
@@ -400,15 +431,11 @@ LYM_DI su3<R> load_link_r(const vec2_t<S> *__restrict__ plane, long V, unsigned 
     if constexpr (ROWS == 3) {
         return load_link<R, S>(plane, V, s);
     } else {
-        using F2 = vec2_t<S>;
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 6 * V * long(sizeof(F2)));
-        const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+        vec2_t<S> v[6];
+        load_elems<S, 6>(plane, V, s, v);
         su3<S> m;
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const F2 v = buf_load<F2>(r, off, unsigned(k) * es);
-            m.e[k] = cplx<S>{v.x, v.y};
-        }
+        for (int k = 0; k < 6; k++) m.e[k] = cplx<S>{v[k].x, v[k].y};
         complete_row2(m);
         return convert<R>(m);
     }
@@ -420,16 +447,13 @@ LYM_DI void store_link_r(vec2_t<S> *__restrict__ plane, long V, unsigned s, cons
     if constexpr (ROWS == 3) {
         store_link<S, R, AUX>(plane, V, s, m);
     } else {
-        using F2 = vec2_t<S>;
-        const __amdgpu_buffer_rsrc_t r = make_rsrc(plane, 6 * V * long(sizeof(F2)));
-        const unsigned off = s * unsigned(sizeof(F2)), es = unsigned(V * sizeof(F2));
+        vec2_t<S> v[6];
 #pragma unroll
         for (int k = 0; k < 6; k++) {
-            F2 v;
-            v.x = S(m.e[k].re);
-            v.y = S(m.e[k].im);
-            buf_store<F2, AUX>(v, r, off, unsigned(k) * es);
+            v[k].x = S(m.e[k].re);
+            v[k].y = S(m.e[k].im);
         }
+        store_elems<S, 6, AUX>(plane, V, s, v);
     }
 }
 
@@ -754,11 +778,15 @@ template <typename R> LYM_DI herm8<double> h8_scale(double a, const herm8<R> &x)
 // o02.re, o02.im, o12.re, o12.im (buffer addressing as load_link)
 template <typename S, int AUX = 0> LYM_DI herm8<S> load_h8(const S *__restrict__ a, long V, unsigned s)
 {
-    const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
     const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
+    constexpr int G = sizeof(S) == 4 ? 8 : 4;  // planes per descriptor: 8 V sizeof(S) < 2^31 bytes in FP32
     S v[8];
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = buf_load<S, AUX>(r, off, unsigned(k) * es);
+    for (int h = 0; h < 8; h += G) {
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(a + long(h) * V, G * V * long(sizeof(S)));
+#pragma unroll
+        for (int k = 0; k < G; k++) v[h + k] = buf_load<S, AUX>(r, off, unsigned(k) * es);
+    }
     herm8<S> z;
     z.d0 = v[0];
     z.d1 = v[1];
@@ -769,11 +797,15 @@ template <typename S, int AUX = 0> LYM_DI herm8<S> load_h8(const S *__restrict__
 }
 template <typename S, int AUX = 0> LYM_DI void store_h8(S *__restrict__ a, long V, unsigned s, const herm8<S> &z)
 {
-    const __amdgpu_buffer_rsrc_t r = make_rsrc(a, 8 * V * long(sizeof(S)));
     const unsigned off = s * unsigned(sizeof(S)), es = unsigned(V * sizeof(S));
+    constexpr int G = sizeof(S) == 4 ? 8 : 4;
     const S v[8] = {z.d0, z.d1, z.o01.re, z.o01.im, z.o02.re, z.o02.im, z.o12.re, z.o12.im};
 #pragma unroll
-    for (int k = 0; k < 8; k++) buf_store<S, AUX>(v[k], r, off, unsigned(k) * es);
+    for (int h = 0; h < 8; h += G) {
+        const __amdgpu_buffer_rsrc_t r = make_rsrc(a + long(h) * V, G * V * long(sizeof(S)));
+#pragma unroll
+        for (int k = 0; k < G; k++) buf_store<S, AUX>(v[h + k], r, off, unsigned(k) * es);
+    }
 }
 
 // Project a near-unitary matrix back onto SU(3): Gram-Schmidt on rows 0, 1,

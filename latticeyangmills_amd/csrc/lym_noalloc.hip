@@ -95,9 +95,10 @@ int lym_observables_noalloc(const void *links, const int dims[4], int precision,
     if (V > LYM_NOALLOC_MAX_SITES)
         return fail(LYM_ERR_TOO_LARGE, "lym_observables_noalloc: more than " + std::to_string(LYM_NOALLOC_MAX_SITES) +
                                            " sites, the maximum its static scratch holds");
-    // one direction plane per buffer descriptor, as lym_create
-    if (9 * V * (precision == LYM_PREC_FP64 ? 16L : 8L) >= (1L << 31))
-        return fail(LYM_ERR_ARG, "lym_observables_noalloc: one direction plane must stay below 2 GiB");
+    // 48 V bytes per buffer descriptor, as lym_create (always true below the
+    // static scratch's maximum; kept for the day that maximum grows)
+    if (48 * V >= (1L << 31))
+        return fail(LYM_ERR_ARG, "lym_observables_noalloc: more than 44.7 M sites");
     g.V = V;
     int dev = -1;
     hipError_t e = hipGetDevice(&dev);

@@ -241,3 +241,37 @@ def test_metropolis_equilibrium_plaquette(candidate, eps, start, therm):
     mean = float(np.mean(plaq))
     print(f"beta 6.0, {candidate} eps {eps}: acceptance {acc:.3f}, <P> = {mean:.6f} (spread {np.std(plaq):.1e})")
     assert abs(mean - 0.5937) < 1e-3
+
+
+def test_64x64x64x64_fp64_equals_tiled_32x32x32x32():
+    """A lattice beyond 2 GiB per direction plane in FP64 (64^4: 16.8 M sites,
+    2.4 GiB per plane): a 32^4 configuration tiled 2 x 2 x 2 x 2 is periodic
+    with the smaller period, so one flow step on 64^4 (whole x-rows in LDS)
+    gives in every copy the 32^4 step (slab staging) bit for bit, and the
+    observables are those of 32^4 (plaquette) and 16 times them (energy, Q)."""
+    small_dims, big_dims = (32, 32, 32, 32), (64, 64, 64, 64)
+    with LatticeContext(small_dims) as ctx:
+        ctx.set_hot(81)
+        ctx.metropolis(6.0, 0.24, 10, 3, 0, "exph", 81)
+        start = ctx.download()
+        obs_small = ctx.observables()
+        ctx.flow_steps(0.02, 1)
+        step_small = ctx.download()
+    shape5 = (4, 32, 32, 32, 32, 18)  # mu, t, z, y, x, element
+    big = np.tile(start.reshape(shape5), (1, 2, 2, 2, 2, 1)).reshape(4, -1, 18)
+    del start
+    with LatticeContext(big_dims) as ctx:
+        ctx.upload(big)
+        obs_big = ctx.observables()
+        ctx.flow_steps(0.02, 1)
+        big = ctx.download()
+    out = big.reshape(4, 64, 64, 64, 64, 18)
+    want = step_small.reshape(shape5)
+    for t0 in (0, 32):
+        for z0 in (0, 32):
+            for y0 in (0, 32):
+                for x0 in (0, 32):
+                    np.testing.assert_array_equal(out[:, t0:t0 + 32, z0:z0 + 32, y0:y0 + 32, x0:x0 + 32], want)
+    assert abs(obs_big[0] / obs_small[0] - 1) < 1e-13
+    assert abs(obs_big[1] / (16 * obs_small[1]) - 1) < 1e-12
+    assert abs(obs_big[2] - 16 * obs_small[2]) < 1e-12 * obs_big[3]

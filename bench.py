@@ -48,9 +48,9 @@ FLOP_PER_LINK_UPDATE = {"mixed": 9244, "fp64": 9631}
 # two rows (12 reals), so it moves fewer bytes than the canonical 18-real
 # count of REALS_PER_LINK_UPDATE.
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
-PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): (1.405e9, PROFILE_FILE), ("mixed-32x64", "fp64"): (6.412e9, PROFILE_FILE),
-                    ("mixed-48x96", "mixed"): (8.694e9, "profiles/k_flow_stage_48x48x48x96.md"),
-                    ("mixed-48x96", "fp64"): (25.60e9, "profiles/k_flow_stage_48x48x48x96.md")}
+PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): (1.406e9, PROFILE_FILE), ("mixed-32x64", "fp64"): (6.372e9, PROFILE_FILE),
+                    ("mixed-48x96", "mixed"): (8.714e9, "profiles/k_flow_stage_48x48x48x96.md"),
+                    ("mixed-48x96", "fp64"): (25.65e9, "profiles/k_flow_stage_48x48x48x96.md")}
 
 WORKLOADS = {
     # 1000 sweeps from a Haar-random start put t0 (t^2 <E> = 0.3) near t = 5.1,

@@ -158,8 +158,9 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
 /* or-ed into a kind, LYM_PREC_MIXED only: the flow calls (lym_flow,
  * lym_flow_steps, lym_profile_flow) keep the links as their first two rows,
  * row 2 = conj(row0 x row1) rebuilt where a kernel reads a link: a third
- * fewer bytes per link, more arithmetic.  A flow call converts the field on
- * entry and leaves it in two-row form, which lym_observables reads as it is;
+ * fewer bytes per link, more arithmetic.  lym_upload_links stores that form
+ * directly, a flow call converts any other field on entry, and the field
+ * stays in two-row form, which lym_observables reads as it is;
  * the calls that read links as three rows (download, staples, derivative,
  * Metropolis, link check, autotune) convert it back first.  lym_autotune
  * tries it per shape. */

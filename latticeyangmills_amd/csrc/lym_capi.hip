@@ -406,18 +406,24 @@ int lym_upload_links(lym_ctx *c, const double *const by_mu[4])
         if (!by_mu[mu]) return fail(c, LYM_ERR_ARG, "null link array");
     LYM_HIP(c, hipSetDevice(c->device));
     const long V = c->g.V;
+    // a mixed context whose flow keeps two-row links takes them in that form
+    // at once (the flow then converts nothing)
+    const int form = c->prec == LYM_PREC_MIXED ? c->rows : 3;
     for (int mu = 0; mu < 4; mu++) {
         LYM_HIP(c, lym::h2d_bounced(c->bounce, c->stream, c->staging, by_mu[mu], sizeof(double) * 18 * V));
         if (c->prec == LYM_PREC_FP64)
             lym::k_aos_to_soa<double><<<blocks_for(V, 256), 256, 0, c->stream>>>(
                 c->staging, static_cast<double2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
+        else if (form == 2)
+            lym::k_aos_to_rows2<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
+                c->staging, static_cast<float2 *>(c->W[c->cur]) + size_t(mu) * 6 * V, V);
         else
             lym::k_aos_to_soa<float><<<blocks_for(V, 256), 256, 0, c->stream>>>(
                 c->staging, static_cast<float2 *>(c->W[c->cur]) + size_t(mu) * 9 * V, V);
         LYM_LAUNCHED(c, "k_aos_to_soa");
     }
     c->eo = false;
-    c->form = 3;
+    c->form = form;
     LYM_HIP(c, hipStreamSynchronize(c->stream));
     return LYM_OK;
 }

@@ -746,6 +746,20 @@ __global__ void k_aos_to_soa(const double *__restrict__ aos, F2<S> *__restrict__
     }
 }
 
+// reference layout (V x 18 doubles for one mu) -> two-row plane of direction
+// mu (rows 0, 1 rounded to S; the row 2 a reader rebuilds is the one an FP32
+// three-row field would hold)
+template <typename S>
+__global__ void k_aos_to_rows2(const double *__restrict__ aos, F2<S> *__restrict__ plane, long V)
+{
+    const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= V) return;
+    su3<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k] = cplx<double>{k < 6 ? aos[s * 18 + 2 * k] : 0.0, k < 6 ? aos[s * 18 + 2 * k + 1] : 0.0};
+    store_link_r<S, double, 0, 2>(plane, V, unsigned(s), m);
+}
+
 template <typename S>
 __global__ void k_soa_to_aos(const F2<S> *__restrict__ plane, double *__restrict__ aos, long V)
 {

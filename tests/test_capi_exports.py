@@ -40,3 +40,19 @@ def test_bindings_reject_wrong_shapes_without_a_gpu():
         LatticeContext.upload(ctx, np.zeros((4, 15, 18)))
     with pytest.raises(ValueError, match="uniforms"):
         LatticeContext.metropolis(ctx, 6.0, 0.24, 2, 1, 0, "exph", 0, np.zeros(10))
+
+
+def test_create_checks_the_lattice_before_the_device():
+    """lym_create rejects bad extents, an unknown precision and lattices of
+    44.7 M sites or more (48 V bytes per buffer descriptor must stay below
+    2^31; 64^3 x 171 is the first of its shape beyond) with LYM_ERR_ARG before
+    it looks for a GPU."""
+    lib = ctypes.CDLL(LIB)
+    lib.lym_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int]
+    ctx = ctypes.c_void_p()
+    for dims in [(64, 64, 64, 171), (1, 4, 4, 4), (4, 4, 4, 0)]:
+        for prec in (0, 1):
+            assert lib.lym_create(ctypes.byref(ctx), (ctypes.c_int * 4)(*dims), prec, 0) == -1, dims
+    assert (1 << 31) // 48 < 64 * 64 * 64 * 171 and 64 * 64 * 64 * 170 < (1 << 31) // 48
+    rc = lib.lym_create(ctypes.byref(ctx), (ctypes.c_int * 4)(2, 2, 2, 2), 7, 0)
+    assert rc == -1  # unknown precision

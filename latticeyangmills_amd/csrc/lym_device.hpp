@@ -254,18 +254,17 @@ template <typename R> LYM_DI void acc_su3_rows01(su3<R> &acc, const su3<R> &m)
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const int k = (j + 1) % 3, l = (j + 2) % 3;
-        // conj(m0k m1l - m0l m1k)
+        // acc row 2 += conj(m0k m1l - m0l m1k), accumulated into acc directly
         const cplx<R> a = m.e[k], b = m.e[3 + l], c = m.e[l], d = m.e[3 + k];
-        R re = a.re * b.re;
+        R re = fma(a.re, b.re, acc.e[6 + j].re);
         re = fma(-a.im, b.im, re);
         re = fma(-c.re, d.re, re);
         re = fma(c.im, d.im, re);
-        R im = a.re * b.im;
-        im = fma(a.im, b.re, im);
-        im = fma(-c.re, d.im, im);
-        im = fma(-c.im, d.re, im);
-        acc.e[6 + j].re += re;
-        acc.e[6 + j].im -= im;
+        R im = fma(-a.re, b.im, acc.e[6 + j].im);
+        im = fma(-a.im, b.re, im);
+        im = fma(c.re, d.im, im);
+        im = fma(c.im, d.re, im);
+        acc.e[6 + j] = cplx<R>{re, im};
     }
 }
 
@@ -476,19 +475,30 @@ struct Nbr {
 // Site indices fit in 32 bits (V < 2^31 / (9 * sizeof(complex)), checked when
 // a lattice is created), so the coordinates come from unsigned 32-bit
 // division rather than the far longer 64-bit sequence.
-LYM_DI Nbr neighbours(const Geom g, long s)
+// From the site's coordinates c = (x, y, z, t):
+LYM_DI Nbr neighbours_at(const Geom g, const int c[4])
 {
     Nbr n;
-    uint32_t r = uint32_t(s);
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-        const uint32_t L = uint32_t(g.L[d]);
-        int c = (d < 3) ? int(r % L) : int(r);
-        if (d < 3) r /= L;
-        n.fwd[d] = (c == g.L[d] - 1) ? -(g.L[d] - 1) * g.stride[d] : g.stride[d];
-        n.bwd[d] = (c == 0) ? (g.L[d] - 1) * g.stride[d] : -g.stride[d];
+        n.fwd[d] = (c[d] == g.L[d] - 1) ? -(g.L[d] - 1) * g.stride[d] : g.stride[d];
+        n.bwd[d] = (c[d] == 0) ? (g.L[d] - 1) * g.stride[d] : -g.stride[d];
     }
     return n;
+}
+// From the site index (three divisions per lane):
+LYM_DI Nbr neighbours(const Geom g, long s)
+{
+    int c[4];
+    uint32_t r = uint32_t(s);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const uint32_t L = uint32_t(g.L[d]);
+        c[d] = int(r % L);
+        r /= L;
+    }
+    c[3] = int(r);
+    return neighbours_at(g, c);
 }
 
 // ------------------------------------------------- su(3) exponential map ---
@@ -555,10 +565,11 @@ constexpr double kSeriesC1Max = 0.5;
 constexpr int kSeriesTerms = 18;
 // Short tier: c1 <= 0.004 (spectral radius <= 0.073) needs 9 terms for a
 // truncation error below 2e-18.  The Wilson flow at eps <= 0.02 is almost
-// always there.  Each lane decides from its own c1: terms 0..9 run for every
-// lane, terms 10..18 only for lanes above the tier (a wave whose lanes are all
-// in the tier skips them), so a link's result depends on its own Q alone and
-// not on which sites share its wave.
+// always there.  Each lane decides from its own c1: lanes in the tier take
+// the short sum of terms 0..9 (short_series), lanes above it the recursion
+// over terms 0..18 (a wave whose lanes are all in the tier skips it), so a
+// link's result depends on its own Q alone and not on which sites share its
+// wave.
 constexpr double kShortSeriesC1Max = 0.004;
 constexpr int kShortSeriesTerms = 9;
 
@@ -593,6 +604,31 @@ template <int N0, int N1> LYM_DI void series_terms(SeriesState &st, double c0, d
     }
 }
 
+// The short tier, terms 0 .. N (N = 6 or 9), written out: with Q^n = a_n + b_n Q
+// + d_n Q^2, (a, b, d)_n = (1,0,0), (0,1,0), (0,0,1), (c0,c1,0), (0,c0,c1),
+// (c0 c1, c1^2, c0), (c0^2, 2 c0 c1, c1^2), (c0 c1^2, c0^2 + c1^3, 2 c0 c1),
+// (2 c0^2 c1, 3 c0 c1^2, c0^2 + c1^3), (c0^3 + c0 c1^3, 3 c0^2 c1 + c1^4, 3 c0 c1^2)
+// for n = 0 .. 9, and f_j = sum_n i^n / n! times the j-th of them, grouped as
+// polynomials in c1 (about a third of the operations of the recursion, whose
+// sums it equals to rounding).
+template <int N> LYM_DI void short_series(double c0, double c1, cplx<double> &f0, cplx<double> &f1, cplx<double> &f2)
+{
+    static_assert(N == 6 || N == 9, "short series of 6 or 9 terms");
+    const double c0sq = c0 * c0;
+    if constexpr (N == 6) {
+        f0 = cplx<double>{fma(c0sq, -1.0 / 720.0, 1.0), c0 * fma(c1, 1.0 / 120.0, -1.0 / 6.0)};
+        f1 = cplx<double>{c0 * fma(c1, -1.0 / 360.0, 1.0 / 24.0), fma(c1, fma(c1, 1.0 / 120.0, -1.0 / 6.0), 1.0)};
+        f2 = cplx<double>{fma(c1, fma(c1, -1.0 / 720.0, 1.0 / 24.0), -0.5), c0 * (1.0 / 120.0)};
+    } else {
+        const double p1 = fma(c1, fma(c1, fma(c1, 1.0 / 362880.0, -1.0 / 5040.0), 1.0 / 120.0), -1.0 / 6.0);
+        f0 = cplx<double>{fma(c0sq, fma(c1, 2.0 / 40320.0, -1.0 / 720.0), 1.0), c0 * fma(c0sq, 1.0 / 362880.0, p1)};
+        f1 = cplx<double>{c0 * fma(c1, fma(c1, 3.0 / 40320.0, -1.0 / 360.0), 1.0 / 24.0),
+                          fma(c0sq, fma(c1, 3.0 / 362880.0, -1.0 / 5040.0), fma(c1, p1, 1.0))};
+        f2 = cplx<double>{fma(c0sq, 1.0 / 40320.0, fma(c1, fma(c1, fma(c1, 1.0 / 40320.0, -1.0 / 720.0), 1.0 / 24.0), -0.5)),
+                          c0 * fma(c1, fma(c1, 3.0 / 362880.0, -2.0 / 5040.0), 1.0 / 120.0)};
+    }
+}
+
 // Links stored in FP32 (the mixed path) carry a rounding of 6e-8, so there the
 // tiers stop earlier: terms 0..6 for c1 <= 0.004 (truncation below 2.2e-12)
 // and 0..14 up to c1 = 0.5 (below 4e-14).
@@ -608,11 +644,17 @@ LYM_DI su3<double> exp_iq_series(const herm8<double> &q, const su3<double> &q2, 
     const double n12 = fma(q.o12.re, q.o12.re, q.o12.im * q.o12.im);
     // c0 = det Q = d0 d1 d2 + 2 Re(o01 o12 conj(o02)) - d0 |o12|^2 - d1 |o02|^2 - d2 |o01|^2
     const double c0 = fma(-d2, n01, fma(-q.d1, n02, fma(-q.d0, n12, fma(q.d0 * q.d1, d2, 2.0 * re3))));
-    SeriesState st{1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    series_terms<0, N_SHORT>(st, c0, c1);
-    if (c1 > kShortSeriesC1Max) series_terms<N_SHORT + 1, N_LONG>(st, c0, c1);
-    return exp_assemble(q, q2, d2, cplx<double>{st.f0r, st.f0i}, cplx<double>{st.f1r, st.f1i},
-                        cplx<double>{st.f2r, st.f2i});
+    cplx<double> f0, f1, f2;
+    if (c1 > kShortSeriesC1Max) {  // long tier: the recursion over terms 0 .. N_LONG
+        SeriesState st{1.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        series_terms<0, N_LONG>(st, c0, c1);
+        f0 = cplx<double>{st.f0r, st.f0i};
+        f1 = cplx<double>{st.f1r, st.f1i};
+        f2 = cplx<double>{st.f2r, st.f2i};
+    } else {
+        short_series<N_SHORT>(c0, c1, f0, f1, f2);
+    }
+    return exp_assemble(q, q2, d2, f0, f1, f2);
 }
 
 // exp(iQ) by the series alone, for Q known to have c1 = Tr Q^2 / 2 <= kSeriesC1Max

@@ -238,12 +238,12 @@ LYM_DI su3<R> staple(const F2<S> *__restrict__ f, long V, long s, const Nbr &n)
 // before any update).  Staple and force run in the storage precision S;
 // the RK combination, the exponential and exp*U run in FP64.  REUNIT
 // projects the result back onto SU(3) before it is stored (mixed path).
+// n: the neighbour offsets of site s.
 template <typename S, int STAGE, bool REUNIT, int MU, typename Src>
 LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ acc, const Geom g, long s,
-                      double eps)
+                      const Nbr &n, double eps)
 {
     const long V = g.V;
-    Nbr n = neighbours(g, s);
     su3<S> om = staple_from<S, MU>(src, s, n);
     su3<S> U = src.template get<S, 0, 0, 0>(MU, s);
     // Z of X = U Omega and the RK combination in the storage precision S
@@ -350,11 +350,14 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         }
         __syncthreads();
         const TileLinks<S, ROWS> src{win, g.V, tile, self};
+        // zt is uniform over the workgroup: z and t cost no per-lane division
+        const int c[4] = {x0 + lx, y0 + ly, int(zt % uint32_t(g.L[2])), int(zt / uint32_t(g.L[2]))};
+        const Nbr n = neighbours_at(g, c);
         switch (mu) {
-        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
-        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
-        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
-        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, n, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, n, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, n, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, n, eps); break;
         }
         __syncthreads();
         return;
@@ -368,7 +371,8 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         // (periodic in y) at slab rows 0 and R + 1.
         F2<S> *const slab = lds;
         const int Lx = g.L[0], Ly = g.L[1], R = kSitesPerBlock / Lx;
-        const int y0 = int(uint32_t(site0) / uint32_t(Lx) % uint32_t(Ly));
+        const uint32_t rows0 = uint32_t(site0) / uint32_t(Lx), zt = rows0 / uint32_t(Ly);  // uniform
+        const int y0 = int(rows0 - zt * uint32_t(Ly));
         const long lo = (y0 > 0) ? site0 - Lx : site0 - Lx + long(Ly) * Lx;
         const long hi = (y0 + R < Ly) ? site0 + kSitesPerBlock : site0 + kSitesPerBlock - long(Ly) * Lx;
         const F2<S> *plane = win + (long)mu * plane_elems<ROWS>() * g.V;
@@ -394,11 +398,13 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         }
         __syncthreads();
         const SlabLinks<S, ROWS> src{win, g.V, slab, Lx - 1, lane >> __builtin_ctz(unsigned(Lx)), lane & (Lx - 1)};
+        const int c[4] = {src.x, y0 + src.row, int(zt % uint32_t(g.L[2])), int(zt / uint32_t(g.L[2]))};
+        const Nbr n = neighbours_at(g, c);
         switch (mu) {
-        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
-        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
-        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
-        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, n, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, n, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, n, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, n, eps); break;
         }
         __syncthreads();
     } else if constexpr (LDS != 0) {
@@ -415,22 +421,24 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         __syncthreads();
         if (active) {
             const RunLinks<S, LDS == 2, ROWS> src{win, g.V, site0, lds};
+            const Nbr n = neighbours(g, s);
             switch (mu) {
-            case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
-            case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
-            case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
-            default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+            case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, n, eps); break;
+            case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, n, eps); break;
+            case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, n, eps); break;
+            default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, n, eps); break;
             }
         }
         __syncthreads();
     } else {
         if (!active) return;
         const GlobalLinks<S, ROWS> src{win, g.V};
+        const Nbr n = neighbours(g, s);
         switch (mu) {
-        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, eps); break;
-        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, eps); break;
-        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, eps); break;
-        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, eps); break;
+        case 0: flow_link<S, STAGE, REUNIT, 0>(src, wout, acc, g, s, n, eps); break;
+        case 1: flow_link<S, STAGE, REUNIT, 1>(src, wout, acc, g, s, n, eps); break;
+        case 2: flow_link<S, STAGE, REUNIT, 2>(src, wout, acc, g, s, n, eps); break;
+        default: flow_link<S, STAGE, REUNIT, 3>(src, wout, acc, g, s, n, eps); break;
         }
     }
 }

@@ -82,11 +82,22 @@ template <typename S, bool XROWS, int ROWS = 3> struct RunLinks {
 
 // Slab staging for runs of R = 64 / Lx whole x-rows (Lx | 64, Lx <= 32, and
 // Ly a multiple of R so that a run never crosses a z-plane): the run's rows
-// and one halo row on each side in y (periodic) are staged in LDS as
-// slab[(dir*9 + k)*kSlabSites + (row + 1)*Lx + x], row = -1 .. R.  Every read
-// whose offset has no z or t part comes from LDS: 10.5 of the 19 reads per
-// link update on average, against 7 with XROWS alone.
-constexpr int kSlabSites = 128;  // (R + 2) * Lx = 64 + 2 Lx <= 128
+// and one halo row in y (periodic) are staged in LDS, per direction the halo
+// row that staple reads use most.  No staple reads U_y one row up (y0 + R),
+// and U_x, U_z, U_t one row down (y0 - 1) only once per site (the lower
+// staple of plane (mu, y) for mu = x, z, t), against two or three reads of
+// the row up; so direction y keeps rows -1 .. R-1 and the others rows 0 .. R,
+// and the reads one row down of x, z and t go to HBM.  Direction d's element
+// planes are kSlabSites sites apart, row r at (r - slab_row0(d)) * Lx:
+//   slab[(d*9 + k)*kSlabSites + (r - slab_row0(d))*Lx + x].
+// 27.6 KB per workgroup in FP32 instead of 36.9 KB with both halo rows of
+// every direction, so five workgroups share a CU instead of four (the FP32
+// slab kernel is compiled for the registers of five waves).  Every
+// other read whose offset has no z or t part comes from LDS: 9.75 of the 19
+// reads per link update on average, against 7 with XROWS alone.
+constexpr int kSlabSites = 96;  // (R + 1) * Lx = 64 + Lx <= 96
+constexpr int slab_row0(int d) { return d == 1 ? -1 : 0; }
+constexpr bool slab_staged(int d, int dy) { return d == 1 ? dy <= 0 : dy >= 0; }
 
 // Whether a lattice takes the slab staging: Lx | 64, Lx <= 32, Ly % (64 / Lx) == 0.
 inline bool slab_fits(const int dims[4])
@@ -100,14 +111,17 @@ template <typename S, int ROWS = 3> struct SlabLinks {
     long V;
     const F2<S> *slab;
     int lx_mask, row, x;  // Lx - 1; this lane's row in the run and x coordinate
+    // dir is a constant at every call site once inlined, so the choice of
+    // LDS or HBM folds away.
     template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
         if constexpr (ZT == 0) {
-            const int li = (row + 1 + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
-            return lds_link<R, S>(slab + dir * 9 * kSlabSites + li, kSlabSites);
-        } else {
-            return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
+            if (slab_staged(dir, DY)) {
+                const int li = (row - slab_row0(dir) + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
+                return lds_link<R, S>(slab + dir * 9 * kSlabSites + li, kSlabSites);
+            }
         }
+        return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
 
@@ -132,12 +146,13 @@ template <typename S> struct EOLinks {
 // Tile staging for lattices whose x-rows do not divide a run (Lx = 48, 80,
 // ...): the 64 sites of a workgroup are a kTileX x kTileY = 16 x 4 block of
 // one (z, t) plane, and the block with a one-site halo ring in x and y
-// (periodic) is staged as tile[(dir*9 + k)*kSlabSites + (ly + 1)*(kTileX + 2)
+// (periodic) is staged as tile[(dir*9 + k)*kTileSites + (ly + 1)*(kTileX + 2)
 // + lx + 1], ly = -1 .. 4, lx = -1 .. 16: 108 sites.  Every read whose offset
 // has no z or t part comes from LDS, as with the slab.  A tile row of one
 // element is a whole 128-byte line in FP32; narrower tiles (8 x 8 for
 // Lx = 24) were measured slower than staging own links only.
 constexpr int kTileX = 16, kTileY = kSitesPerBlock / kTileX;
+constexpr int kTileSites = 128;  // (kTileX + 2) * (kTileY + 2) = 108, rounded up
 inline bool tile_fits(const int dims[4]) { return dims[0] % kTileX == 0 && dims[1] % kTileY == 0; }
 template <typename S, int ROWS = 3> struct TileLinks {
     static constexpr int rows = ROWS;
@@ -148,7 +163,7 @@ template <typename S, int ROWS = 3> struct TileLinks {
     template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
         if constexpr (ZT == 0)
-            return lds_link<R, S>(tile + dir * 9 * kSlabSites + self + DY * (kTileX + 2) + DX, kSlabSites);
+            return lds_link<R, S>(tile + dir * 9 * kTileSites + self + DY * (kTileX + 2) + DX, kTileSites);
         else
             return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
@@ -285,8 +300,13 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
 // picks the mode per workgroup passes every mode the same array.
 template <int LDS> constexpr int stage_lds_elems()
 {
-    return LDS == 0 ? 1 : (LDS == 3 || LDS == 5) ? 4 * 9 * kSlabSites : 4 * 9 * kSitesPerBlock;
+    return LDS == 0 ? 1 : LDS == 3 ? 4 * 9 * kSlabSites : LDS == 5 ? 4 * 9 * kTileSites : 4 * 9 * kSitesPerBlock;
 }
+// Waves per SIMD the register allocation of a stage kernel must allow: the
+// FP32 slab kernel fits five workgroups' LDS on a CU, so it is held to the
+// registers of five waves; the others take what they need.
+template <typename S, int LDS> constexpr int stage_min_waves() { return sizeof(S) == 4 && LDS == 3 ? 5 : 1; }
+
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
                        const Geom g, long site0, double eps, F2<S> *__restrict__ lds)
@@ -310,14 +330,14 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         const int lx = lane % TX, ly = lane / TX;
         const long s = pb + x0 + lx + (long)Lx * (y0 + ly);
         const F2<S> *plane = win + (long)mu * plane_elems<ROWS>() * g.V;
-        F2<S> *dst = tile + mu * 9 * kSlabSites;
+        F2<S> *dst = tile + mu * 9 * kTileSites;
         const int self = (ly + 1) * W + lx + 1;
         {
             const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(s));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                dst[k * kSlabSites + self].x = u.e[k].re;
-                dst[k * kSlabSites + self].y = u.e[k].im;
+                dst[k * kTileSites + self].x = u.e[k].re;
+                dst[k * kTileSites + self].y = u.e[k].im;
             }
         }
         if (lane < 2 * W + 2 * TY) {
@@ -344,8 +364,8 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
             const int li = (hy + 1) * W + hx + 1;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                dst[k * kSlabSites + li].x = u.e[k].re;
-                dst[k * kSlabSites + li].y = u.e[k].im;
+                dst[k * kTileSites + li].x = u.e[k].re;
+                dst[k * kTileSites + li].y = u.e[k].im;
             }
         }
         __syncthreads();
@@ -367,8 +387,8 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
     if constexpr (LDS == 3) {
         // Slab: runs are whole x-rows and never cross a z-plane, so every lane
         // is a lattice site (checked on the host).  Wave mu stages direction mu:
-        // the run's rows at slab rows 1..R, and the rows y0 - 1 and y0 + R
-        // (periodic in y) at slab rows 0 and R + 1.
+        // the run's rows 0 .. R-1, and of the rows y0 - 1 and y0 + R (periodic
+        // in y) those that direction keeps (SlabLinks).
         F2<S> *const slab = lds;
         const int Lx = g.L[0], Ly = g.L[1], R = kSitesPerBlock / Lx;
         const uint32_t rows0 = uint32_t(site0) / uint32_t(Lx), zt = rows0 / uint32_t(Ly);  // uniform
@@ -377,18 +397,19 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
         const long hi = (y0 + R < Ly) ? site0 + kSitesPerBlock : site0 + kSitesPerBlock - long(Ly) * Lx;
         const F2<S> *plane = win + (long)mu * plane_elems<ROWS>() * g.V;
         F2<S> *dst = slab + mu * 9 * kSlabSites;
+        const int r0 = slab_row0(mu);
         {
             const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(s));
+            const int li = -r0 * Lx + lane;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                dst[k * kSlabSites + Lx + lane].x = u.e[k].re;
-                dst[k * kSlabSites + Lx + lane].y = u.e[k].im;
+                dst[k * kSlabSites + li].x = u.e[k].re;
+                dst[k * kSlabSites + li].y = u.e[k].im;
             }
         }
-        if (lane < 2 * Lx) {
-            const bool below = lane < Lx;
-            const long hs = below ? lo + lane : hi + (lane - Lx);
-            const int li = below ? lane : (R + 1) * Lx + (lane - Lx);
+        if (lane < Lx) {  // the halo row: y0 - 1 for direction y, y0 + R for the others
+            const long hs = (mu == 1 ? lo : hi) + lane;
+            const int li = (mu == 1 ? 0 : R * Lx) + lane;
             const su3<S> u = load_link_r<S, S, ROWS>(plane, g.V, unsigned(hs));
 #pragma unroll
             for (int k = 0; k < 9; k++) {
@@ -444,8 +465,9 @@ LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, 
 }
 
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
-__global__ __launch_bounds__(256) void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
-                                                    S *__restrict__ acc, Geom g, Sched sc, double eps)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(stage_min_waves<S, LDS>())))
+void k_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc, Geom g, Sched sc,
+                  double eps)
 {
     const long run = sched_run(sc, blockIdx.x);  // uniform over the workgroup
     if (run < 0) return;

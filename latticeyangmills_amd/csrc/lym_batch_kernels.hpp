@@ -38,37 +38,52 @@ LYM_DI int find_conf(const ConfDesc *__restrict__ d, int n, long tile)
     return lo;
 }
 
-// The grid strides over the schedule slots: with one workgroup per slot the
-// loop body runs once, with fewer workgroups (a persistent grid, a multiple
-// of 8 so that a workgroup keeps its XCD) each takes every gridDim.x-th slot.
-// LDS as flow_block, and 4: per lattice, slab staging where the lattice fits
-// it and the run's own links elsewhere (the lattice of a tile, and so the
-// choice, is uniform over the workgroup); both modes stage into one array.
-// ROWS = 2: the fields are in two-row form, 6 complexes per link, so a
-// lattice's field starts at link_off / 9 * 6.
+// One schedule slot of a batch RK stage: the tile it holds (none for padding
+// entries), its lattice, and the lattice's staging mode.  LDS as flow_block,
+// and 4: per lattice, slab staging where the lattice fits it and the run's
+// own links elsewhere (the lattice of a tile, and so the choice, is uniform
+// over the workgroup); both modes stage into one array.  ROWS = 2: the fields
+// are in two-row form, 6 complexes per link, so a lattice's field starts at
+// link_off / 9 * 6.
 template <int ROWS> LYM_DI long field_off(const ConfDesc &d) { return ROWS == 2 ? d.link_off / 9 * 6 : d.link_off; }
 
-template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
-__global__ __launch_bounds__(256) void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
-                                                          S *__restrict__ acc, const ConfDesc *__restrict__ desc,
-                                                          int nconf, long ntiles, Sched sc, double eps)
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS>
+LYM_DI void batch_slot(long k, const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
+                       const ConfDesc *__restrict__ desc, int nconf, Sched sc, double eps, F2<S> *lds)
+{
+    const long tile = sched_run(sc, unsigned(k));  // uniform over the workgroup
+    if (tile < 0) return;
+    const int c = find_conf(desc, nconf, tile);
+    const ConfDesc d = desc[c];
+    const long site0 = (tile - d.tile0) * kSitesPerBlock, off = field_off<ROWS>(d);
+    if constexpr (LDS == 4) {
+        if (d.slab)
+            flow_block<S, STAGE, REUNIT, 3, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+        else
+            flow_block<S, STAGE, REUNIT, 1, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+    } else {
+        flow_block<S, STAGE, REUNIT, LDS, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+    }
+}
+
+// PERSIST = false: one workgroup per schedule slot.  PERSIST = true: a
+// persistent grid (fewer workgroups, a multiple of 8 so that a workgroup keeps
+// its XCD) strides over the slots.  The loop keeps values live across
+// iterations and costs the mixed kernel some 50 registers (3 instead of 5
+// waves per SIMD), so the one-slot form is a kernel of its own.
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3, bool PERSIST = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : stage_min_waves<S, LDS>())))
+void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
+                        const ConfDesc *__restrict__ desc, int nconf, long ntiles, Sched sc, double eps)
 {
     const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
     __shared__ F2<S> lds[stage_lds_elems<LDS == 4 ? 3 : LDS>()];
-    for (long k = blockIdx.x; k < slots; k += gridDim.x) {
-        const long tile = sched_run(sc, unsigned(k));  // uniform over the workgroup
-        if (tile < 0) continue;
-        const int c = find_conf(desc, nconf, tile);
-        const ConfDesc d = desc[c];
-        const long site0 = (tile - d.tile0) * kSitesPerBlock, off = field_off<ROWS>(d);
-        if constexpr (LDS == 4) {
-            if (d.slab)
-                flow_block<S, STAGE, REUNIT, 3, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
-            else
-                flow_block<S, STAGE, REUNIT, 1, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
-        } else {
-            flow_block<S, STAGE, REUNIT, LDS, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
-        }
+    if constexpr (PERSIST) {
+        for (long k = blockIdx.x; k < slots; k += gridDim.x)
+            batch_slot<S, STAGE, REUNIT, LDS, ROWS>(k, win, wout, acc, desc, nconf, sc, eps, lds);
+    } else {
+        if (long(blockIdx.x) < slots)
+            batch_slot<S, STAGE, REUNIT, LDS, ROWS>(blockIdx.x, win, wout, acc, desc, nconf, sc, eps, lds);
     }
 }
 

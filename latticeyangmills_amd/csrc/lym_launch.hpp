@@ -41,20 +41,42 @@ void launch_flow_stage_r2(int lds, unsigned nb, hipStream_t st, const F2<float> 
     }
 }
 
-// One RK stage over every lattice of a batch: grid workgroups stride over the
-// schedule slots; lds = 0, 1, 2, 3 or 4 (per-lattice slab or own links).
+// One RK stage over every lattice of a batch: one workgroup per schedule slot
+// when grid covers them, else a persistent grid of that many workgroups
+// striding over the slots (k_batch_flow_stage<..., PERSIST>); lds = 0, 1, 2, 3
+// or 4 (per-lattice slab or own links).
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS>
+void launch_batch_stage_lds(unsigned grid, hipStream_t st, const F2<S> *win, F2<S> *wout, S *acc, const ConfDesc *desc,
+                            int nconf, long ntiles, Sched sc, double eps)
+{
+    const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
+    if (long(grid) < slots)
+        k_batch_flow_stage<S, STAGE, REUNIT, LDS, ROWS, true><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles,
+                                                                                   sc, eps);
+    else
+        k_batch_flow_stage<S, STAGE, REUNIT, LDS, ROWS, false><<<unsigned(slots), 256, 0, st>>>(win, wout, acc, desc,
+                                                                                               nconf, ntiles, sc, eps);
+}
+
+template <typename S, int STAGE, int ROWS>
+void launch_batch_stage_rows(int lds, unsigned grid, hipStream_t st, const F2<S> *win, F2<S> *wout, S *acc,
+                             const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps)
+{
+    constexpr bool re = sizeof(S) == 4;  // mixed path: reunitarise before the store
+    switch (lds) {
+    case 0: launch_batch_stage_lds<S, STAGE, re, 0, ROWS>(grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 1: launch_batch_stage_lds<S, STAGE, re, 1, ROWS>(grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 2: launch_batch_stage_lds<S, STAGE, re, 2, ROWS>(grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    case 3: launch_batch_stage_lds<S, STAGE, re, 3, ROWS>(grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    default: launch_batch_stage_lds<S, STAGE, re, 4, ROWS>(grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps); break;
+    }
+}
+
 template <typename S, int STAGE>
 void launch_batch_stage(int lds, unsigned grid, hipStream_t st, const F2<S> *win, F2<S> *wout, S *acc,
                         const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps)
 {
-    constexpr bool re = sizeof(S) == 4;
-    switch (lds) {
-    case 0: k_batch_flow_stage<S, STAGE, re, 0><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 1: k_batch_flow_stage<S, STAGE, re, 1><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 2: k_batch_flow_stage<S, STAGE, re, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 3: k_batch_flow_stage<S, STAGE, re, 3><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    default: k_batch_flow_stage<S, STAGE, re, 4><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    }
+    launch_batch_stage_rows<S, STAGE, 3>(lds, grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps);
 }
 
 // The mixed batch stage over fields in two-row form (ROWS = 2; lym_bstage_f32r2.hip).
@@ -62,13 +84,7 @@ template <int STAGE>
 void launch_batch_stage_r2(int lds, unsigned grid, hipStream_t st, const F2<float> *win, F2<float> *wout, float *acc,
                            const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps)
 {
-    switch (lds) {
-    case 0: k_batch_flow_stage<float, STAGE, true, 0, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 1: k_batch_flow_stage<float, STAGE, true, 1, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 2: k_batch_flow_stage<float, STAGE, true, 2, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    case 3: k_batch_flow_stage<float, STAGE, true, 3, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    default: k_batch_flow_stage<float, STAGE, true, 4, 2><<<grid, 256, 0, st>>>(win, wout, acc, desc, nconf, ntiles, sc, eps); break;
-    }
+    launch_batch_stage_rows<float, STAGE, 2>(lds, grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps);
 }
 
 // Candidate kind of a Metropolis launch: 0 = R S T product, 1 = exp(i eps H)

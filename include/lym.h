@@ -167,9 +167,10 @@ int lym_profile_flow(lym_ctx *ctx, double eps, int nsteps, int measure_every, do
 #define LYM_SCHED_TWO_ROW 32
 int lym_set_schedule(lym_ctx *ctx, int kind, int target);
 /* Times every candidate schedule on the context's own field (RK stage 1,
- * which leaves the field unchanged) and keeps the fastest; cache_path (may be
- * NULL) holds one line per lattice shape and precision; summary receives the
- * choice. */
+ * which leaves the field unchanged; three interleaved rounds, each candidate
+ * keeping its fastest time) and keeps the default schedule unless another is
+ * faster by more than 1.5 %; cache_path (may be NULL) holds one line per
+ * lattice shape and precision; summary receives the choice. */
 int lym_autotune(lym_ctx *ctx, const char *cache_path, char *summary, int summary_len);
 
 /* No counterpart in the reference: SuperObs::compute (src/Observables/superobs.cpp:31-128)

@@ -54,7 +54,9 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
  * grid = 0 means one workgroup per schedule slot, otherwise a persistent grid
  * of that many workgroups (rounded up to a multiple of 8). */
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target);
-/* Times the candidate shapes on the batch itself and keeps the fastest;
+/* Times the candidate shapes on the batch itself and keeps the fastest, as
+ * lym_autotune (the default schedule unless another is faster by more than
+ * 1.5 %; then persistent grids of the chosen order under the same rule);
  * cache_path (may be NULL) holds one line per batch shape and precision.
  * summary receives a one-line description of the choice. */
 int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len);

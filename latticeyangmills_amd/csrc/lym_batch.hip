@@ -312,8 +312,8 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
         return bail(LYM_ERR_HIP);
     if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
     // as the single-lattice default: the mixed flow in two-row form
-    const int kind = lym::SCHED_COLUMN | lym::SCHED_LDS | (precision == LYM_PREC_MIXED ? lym::SCHED_TWO_ROW : 0);
-    if (apply_schedule(b, 0, kind, 4096) != LYM_OK)
+    if (apply_schedule(b, 0, lym::default_schedule_kind(precision == LYM_PREC_MIXED), lym::kDefaultColumnTarget) !=
+        LYM_OK)
         return bail(LYM_ERR_NOMEM);
     // cold start
     for (int c = 0; c < nconf; c++) {
@@ -609,12 +609,13 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
     return LYM_OK;
 }
 
-// Times candidate launch shapes of the RK stage kernel on the batch itself
-// (stage 1 reads W[cur] and writes W[1-cur] and the accumulator, so the field
-// is left unchanged) and keeps the fastest.  Candidates: every schedule of
-// lym::schedule_candidates() with one workgroup per slot, and the best of
-// those with persistent grids of {2, 4, 8} x 256 workgroups.  Results are
-// cached in `cache_path` (one line per batch shape) when it is non-null.
+// Times candidate launch shapes of the RK stage kernel over flow steps of the
+// batch itself (its fields saved first and put back afterwards) and keeps the
+// fastest (lym::tune_rounds).  Candidates:
+// every schedule of lym::schedule_candidates() with one workgroup per slot,
+// then the best of those with persistent grids of {2, 4, 8} x 256 workgroups.
+// Results are cached in `cache_path` (one line per batch shape) when it is
+// non-null.
 int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len)
 {
     if (!b) return LYM_ERR_ARG;
@@ -637,76 +638,72 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
         sscanf(it->second.c_str(), "%d %d %d %lf", &best_grid, &best_kind, &best_target, &best_ms) == 4) {
         b->tuned = "cached";
     } else {
-        hipEvent_t e0, e1;
-        BHIP(b, hipEventCreate(&e0));
-        BHIP(b, hipEventCreate(&e1));
-        const bool mixed = b->prec == LYM_PREC_MIXED;
-        // The two-row candidates read a two-row copy of every lattice and write
-        // a scratch field; both live only while tuning (the field stays as it is).
-        struct Scratch {
-            float2 *p[2] = {nullptr, nullptr};
-            ~Scratch()
+        struct Events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events()
             {
-                for (float2 *q : p)
-                    if (q) (void)hipFree(q);
+                for (hipEvent_t x : e)
+                    if (x) (void)hipEventDestroy(x);
             }
-        } r2;
-        if (mixed) {
-            for (auto &q : r2.p) BHIP(b, hipMalloc(&q, sizeof(float2) * 6 * size_t(b->total_links)));
-            for (const ConfDesc &d : b->desc)
-                lym::k_to_rows2<float><<<dim3(nblocks(d.g.V, 256), 4), 256, 0, b->stream>>>(
-                    static_cast<const float2 *>(b->W[b->cur]) + d.link_off, r2.p[0] + d.link_off / 9 * 6, d.g.V);
-            BLAUNCHED(b, "k_to_rows2");
-        }
-        auto time_stage1 = [&](float *out) -> int {
-            float best_c = 1e30f;
-            for (int rep = 0; rep < 4; rep++) {
-                BHIP(b, hipEventRecord(e0, b->stream));
-                if (b->prec == LYM_PREC_FP64) {
-                    launch_stage<double, 1>(b, static_cast<double2 *>(b->W[b->cur]),
-                                            static_cast<double2 *>(b->W[1 - b->cur]), static_cast<double *>(b->acc), 0.0);
-                } else if (b->rows == 2) {
-                    b->form = 2;
-                    launch_stage<float, 1>(b, r2.p[0], r2.p[1], static_cast<float *>(b->acc), 0.0);
-                    b->form = 3;
-                } else {
-                    launch_stage<float, 1>(b, static_cast<float2 *>(b->W[b->cur]), static_cast<float2 *>(b->W[1 - b->cur]),
-                                           static_cast<float *>(b->acc), 0.0);
-                }
-                BLAUNCHED(b, "k_batch_flow_stage");
-                BHIP(b, hipEventRecord(e1, b->stream));
-                BHIP(b, hipEventSynchronize(e1));
-                float ms = 0;
-                BHIP(b, hipEventElapsedTime(&ms, e0, e1));
-                if (rep > 0) best_c = std::min(best_c, ms);  // first run warms caches
+        } ev;
+        BHIP(b, hipEventCreate(&ev.e[0]));
+        BHIP(b, hipEventCreate(&ev.e[1]));
+        hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1];
+        const bool mixed = b->prec == LYM_PREC_MIXED;
+        // Each candidate is timed over whole RK steps of the batch flow itself
+        // (its three stage launches, in the form the candidate keeps the links
+        // in), so that the pick holds for a flow.  The fields, three rows in
+        // lexicographic order after to_lex() above, are saved first and put
+        // back afterwards.
+        const size_t bytes = (b->prec == LYM_PREC_FP64 ? sizeof(double) : sizeof(float)) * 18 * size_t(b->total_links);
+        struct Save {
+            void *p = nullptr;
+            ~Save()
+            {
+                if (p) (void)hipFree(p);
             }
-            *out = best_c;
+        } save;
+        if (int rc = to_form(b, 3)) return rc;
+        BHIP(b, hipMalloc(&save.p, bytes));
+        BHIP(b, hipMemcpyAsync(save.p, b->W[b->cur], bytes, hipMemcpyDeviceToDevice, b->stream));
+        // an untimed, then a timed flow step under the schedule just applied
+        auto time_step = [&](float *ms) -> int {
+            if (int rc = to_form(b, b->rows)) return rc;
+            if (int rc = step(b, lym::kTuneEps)) return rc;
+            BHIP(b, hipEventRecord(e0, b->stream));
+            if (int rc = step(b, lym::kTuneEps)) return rc;
+            BHIP(b, hipEventRecord(e1, b->stream));
+            BHIP(b, hipEventSynchronize(e1));
+            BHIP(b, hipEventElapsedTime(ms, e0, e1));
             return LYM_OK;
         };
-        int rc;
-        for (auto cd : lym::schedule_candidates(mixed)) {
-            if ((rc = apply_schedule(b, 0, cd.first, cd.second))) return rc;
-            float ms;
-            if ((rc = time_stage1(&ms))) return rc;
-            if (best_ms < 0 || ms < best_ms) {
-                best_ms = ms;
-                best_grid = 0;
-                best_kind = cd.first;
-                best_target = cd.second;
-            }
+        int rc, best = 0;
+        // one workgroup per slot, every order ...
+        const auto cands = lym::schedule_candidates(mixed);
+        auto time_order = [&](int i, float *ms) -> int {
+            if (int rc = apply_schedule(b, 0, cands[size_t(i)].first, cands[size_t(i)].second)) return rc;
+            return time_step(ms);
+        };
+        rc = lym::tune_rounds(int(cands.size()), 3, time_order, &best, &best_ms);
+        if (!rc) {
+            best_grid = 0;
+            best_kind = cands[size_t(best)].first;
+            best_target = cands[size_t(best)].second;
+            // ... then the best order again with persistent grids of {2, 4, 8} x 256 workgroups
+            const int grids[4] = {0, 512, 1024, 2048};
+            auto time_grid = [&](int i, float *ms) -> int {
+                if (int rc = apply_schedule(b, grids[i], best_kind, best_target)) return rc;
+                return time_step(ms);
+            };
+            rc = lym::tune_rounds(4, 3, time_grid, &best, &best_ms);
+            best_grid = grids[best];
         }
-        const int k0 = best_kind, t0 = best_target;
-        for (int m : {2, 4, 8}) {
-            if ((rc = apply_schedule(b, 256 * m, k0, t0))) return rc;
-            float ms;
-            if ((rc = time_stage1(&ms))) return rc;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_grid = 256 * m;
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        // the saved fields back, whatever the outcome
+        const hipError_t e = hipMemcpyAsync(b->W[b->cur], save.p, bytes, hipMemcpyDeviceToDevice, b->stream);
+        b->form = 3;
+        if (rc) return rc;
+        BHIP(b, e);
+        BHIP(b, hipStreamSynchronize(b->stream));
         b->tuned = "measured";
         if (cache_path) {
             std::ofstream o(cache_path, std::ios::app);
@@ -716,7 +713,7 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
     int rc = apply_schedule(b, best_grid, best_kind, best_target);
     if (rc) return rc;
     if (summary && summary_len > 0)
-        snprintf(summary, size_t(summary_len), "grid=%d schedule=%s target=%d stage1_ms=%.4f source=%s tiles=%ld",
+        snprintf(summary, size_t(summary_len), "grid=%d schedule=%s target=%d step_ms=%.4f source=%s tiles=%ld",
                  best_grid, lym::schedule_name(best_kind), best_target, best_ms, b->tuned.c_str(), b->ntiles);
     return LYM_OK;
 }

@@ -355,9 +355,8 @@ int lym_create(lym_ctx **out, const int dims[4], int precision, int device)
     if (hipMalloc(&c->accepted, sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&c->check, 2 * sizeof(unsigned long long)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (ensure_slots(c, 16) != LYM_OK) return bail(LYM_ERR_NOMEM);
-    // the schedule the autotuner picks at 32^3x64 (the mixed flow in two-row form)
-    const int kind = lym::SCHED_COLUMN | lym::SCHED_LDS | (precision == LYM_PREC_MIXED ? lym::SCHED_TWO_ROW : 0);
-    if (apply_schedule(c, kind, 4096) != LYM_OK)
+    // the default schedule (the mixed flow in two-row form), the autotuner's first candidate
+    if (apply_schedule(c, lym::default_schedule_kind(precision == LYM_PREC_MIXED), lym::kDefaultColumnTarget) != LYM_OK)
         return bail(LYM_ERR_NOMEM);
     int rc = lym_set_cold(c);
     if (rc != LYM_OK) return bail(rc);
@@ -806,51 +805,43 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
         int rc = ensure_events(c, 2);
         if (rc) return rc;
         const bool mixed = c->prec == LYM_PREC_MIXED;
-        // The two-row candidates read a two-row copy of the field and write a
-        // scratch field; both live only while tuning (the field stays as it is).
-        struct Scratch {
-            float2 *p[2] = {nullptr, nullptr};
-            ~Scratch()
+        // Each candidate is timed over whole RK steps of the flow itself (its
+        // three stage launches, in the form the candidate keeps the links in),
+        // so that the pick holds for a flow.  The field, three rows in
+        // lexicographic order after readable() above, is saved first and put
+        // back afterwards.
+        const size_t bytes = 2 * real_size(c) * 9 * 4 * size_t(c->g.V);
+        struct Save {
+            void *p = nullptr;
+            ~Save()
             {
-                for (float2 *q : p)
-                    if (q) (void)hipFree(q);
+                if (p) (void)hipFree(p);
             }
-        } r2;
-        if (mixed) {
-            for (auto &q : r2.p) LYM_HIP(c, hipMalloc(&q, sizeof(float2) * 24 * size_t(c->g.V)));
-            lym::k_to_rows2<float><<<dim3(blocks_for(c->g.V, 256), 4), 256, 0, c->stream>>>(
-                static_cast<const float2 *>(c->W[c->cur]), r2.p[0], c->g.V);
-            LYM_LAUNCHED(c, "k_to_rows2");
-        }
-        for (auto cand : lym::schedule_candidates(mixed)) {
-            if ((rc = apply_schedule(c, cand.first, cand.second))) return rc;
-            float best_c = 1e30f;
-            for (int rep = 0; rep < 4; rep++) {  // stage 1 leaves W[cur] unchanged
-                LYM_HIP(c, hipEventRecord(c->events[0], c->stream));
-                if (c->prec == LYM_PREC_FP64) {
-                    launch_stage<double, 1>(c, static_cast<double2 *>(c->W[c->cur]),
-                                            static_cast<double2 *>(c->W[1 - c->cur]), static_cast<double *>(c->acc), 0.0);
-                } else if (c->rows == 2) {
-                    c->form = 2;
-                    launch_stage<float, 1>(c, r2.p[0], r2.p[1], static_cast<float *>(c->acc), 0.0);
-                    c->form = 3;
-                } else {
-                    launch_stage<float, 1>(c, static_cast<float2 *>(c->W[c->cur]), static_cast<float2 *>(c->W[1 - c->cur]),
-                                           static_cast<float *>(c->acc), 0.0);
-                }
-                LYM_LAUNCHED(c, "k_flow_stage");
-                LYM_HIP(c, hipEventRecord(c->events[1], c->stream));
-                LYM_HIP(c, hipEventSynchronize(c->events[1]));
-                float ms = 0;
-                LYM_HIP(c, hipEventElapsedTime(&ms, c->events[0], c->events[1]));
-                if (rep > 0) best_c = std::min(best_c, ms);
-            }
-            if (best_ms < 0 || best_c < best_ms) {
-                best_ms = best_c;
-                best_kind = cand.first;
-                best_target = cand.second;
-            }
-        }
+        } save;
+        LYM_HIP(c, hipMalloc(&save.p, bytes));
+        LYM_HIP(c, hipMemcpyAsync(save.p, c->W[c->cur], bytes, hipMemcpyDeviceToDevice, c->stream));
+        const auto cands = lym::schedule_candidates(mixed);
+        auto time_one = [&](int i, float *ms) -> int {  // an untimed, then a timed flow step
+            if (int rc = apply_schedule(c, cands[size_t(i)].first, cands[size_t(i)].second)) return rc;
+            if (int rc = to_form(c, c->rows)) return rc;
+            if (int rc = step(c, lym::kTuneEps)) return rc;
+            LYM_HIP(c, hipEventRecord(c->events[0], c->stream));
+            if (int rc = step(c, lym::kTuneEps)) return rc;
+            LYM_HIP(c, hipEventRecord(c->events[1], c->stream));
+            LYM_HIP(c, hipEventSynchronize(c->events[1]));
+            LYM_HIP(c, hipEventElapsedTime(ms, c->events[0], c->events[1]));
+            return LYM_OK;
+        };
+        int best = 0;
+        rc = lym::tune_rounds(int(cands.size()), 3, time_one, &best, &best_ms);
+        // the saved field back, whatever the outcome
+        const hipError_t e = hipMemcpyAsync(c->W[c->cur], save.p, bytes, hipMemcpyDeviceToDevice, c->stream);
+        c->form = 3;
+        if (rc) return rc;
+        LYM_HIP(c, e);
+        LYM_HIP(c, hipStreamSynchronize(c->stream));
+        best_kind = cands[size_t(best)].first;
+        best_target = cands[size_t(best)].second;
         if (cache_path) {
             std::ofstream o(cache_path, std::ios::app);
             o << key.str() << "\t" << best_kind << " " << best_target << " " << best_ms << "\n";
@@ -859,7 +850,7 @@ int lym_autotune(lym_ctx *c, const char *cache_path, char *summary, int summary_
     int rc = apply_schedule(c, best_kind, best_target);
     if (rc) return rc;
     if (summary && summary_len > 0)
-        snprintf(summary, size_t(summary_len), "schedule=%s target=%d stage1_ms=%.4f source=%s",
+        snprintf(summary, size_t(summary_len), "schedule=%s target=%d step_ms=%.4f source=%s",
                  lym::schedule_name(best_kind), best_target, best_ms, source);
     return LYM_OK;
 }

@@ -150,21 +150,68 @@ inline HostSchedule metro_column_schedule(const std::array<int, 4> &L, int targe
     return h;
 }
 
+// The schedule a context takes untuned: columns of 4096 sites with LDS
+// staging, in two-row form on the mixed path.
+constexpr int kDefaultColumnTarget = 4096;
+inline int default_schedule_kind(bool two_row) { return SCHED_COLUMN | SCHED_LDS | (two_row ? SCHED_TWO_ROW : 0); }
+
 // the candidates the autotuners time: every order, with and without LDS
-// staging, and (two_row: the mixed single-lattice path) each of these again
-// with the links in two-row form
+// staging, and (two_row: the mixed path) each of these again with the links
+// in two-row form; the default schedule comes first
 inline std::vector<std::pair<int, int>> schedule_candidates(bool two_row = false)
 {
-    std::vector<std::pair<int, int>> c;
+    const std::pair<int, int> def{default_schedule_kind(two_row), kDefaultColumnTarget};
+    std::vector<std::pair<int, int>> c{def};
     const std::vector<std::pair<int, int>> orders{{SCHED_LINEAR, 0},    {SCHED_CONTIG, 0},    {SCHED_COLUMN, 256},
                                                   {SCHED_COLUMN, 512},  {SCHED_COLUMN, 1024}, {SCHED_COLUMN, 2048},
                                                   {SCHED_COLUMN, 4096}, {SCHED_COLUMN, 8192}};
     for (int form : {0, SCHED_TWO_ROW}) {
         if (form && !two_row) break;
         for (int lds : {0, SCHED_LDS})
-            for (auto k : orders) c.push_back({k.first | lds | form, k.second});
+            for (auto k : orders)
+                if (std::make_pair(k.first | lds | form, k.second) != def) c.push_back({k.first | lds | form, k.second});
     }
     return c;
+}
+
+// A timed candidate replaces the best so far only when it is faster by more
+// than this fraction: candidates within the timing noise of each other (the
+// stage times of the column and linear orders often differ by under 1 %) then
+// resolve to the one timed first, so that repeated tunings of a shape pick
+// the same schedule.
+constexpr double kTuneMargin = 0.015;
+// Step size of the flow steps the autotuners time: a typical one, so that
+// the exponential takes the branches it takes in a flow.
+constexpr double kTuneEps = 0.02;
+inline bool tune_better(double ms, double best_ms) { return best_ms < 0 || ms < best_ms * (1.0 - kTuneMargin); }
+
+// Picks one of n candidates by timing.  time_one(i, &ms) applies candidate i
+// and times one flow step after an untimed one (returns 0 or an error code);
+// the untimed step takes the first touch of the schedule table that applying
+// a schedule reallocates, which would otherwise weigh on every order but the
+// linear one (no table).  After one untimed call for candidate 0 (the device
+// warms up), every candidate is timed once per round for `rounds` rounds and
+// keeps its fastest time, so that a drift of the clocks over the tuning does
+// not favour the candidates timed last; then the first candidate in order is
+// kept unless a later one is faster by more than kTuneMargin.
+template <typename F> int tune_rounds(int n, int rounds, F &&time_one, int *best, double *best_ms)
+{
+    std::vector<float> t(size_t(n), 1e30f);
+    float ms = 0;
+    if (int rc = time_one(0, &ms)) return rc;
+    for (int r = 0; r < rounds; r++)
+        for (int i = 0; i < n; i++) {
+            if (int rc = time_one(i, &ms)) return rc;
+            t[size_t(i)] = std::min(t[size_t(i)], ms);
+        }
+    *best = 0;
+    *best_ms = t[0];
+    for (int i = 1; i < n; i++)
+        if (tune_better(t[size_t(i)], *best_ms)) {
+            *best = i;
+            *best_ms = t[size_t(i)];
+        }
+    return 0;
 }
 
 inline const char *schedule_name(int kind)

@@ -38,19 +38,32 @@ STAGE_REALS = (44, 52, 44)
 # + 2 FMA), FP32 and FP64 summed; per-stage figures in PROFILE_FILE): the
 # mixed path (two-row staples, links in two-row form with row 2 rebuilt where
 # read) 3.08 kflop per link per stage, FP64 (all three rows multiplied out)
-# 3.21 kflop.
-FLOP_PER_LINK_UPDATE = {"mixed": 9244, "fp64": 9631}
+# 3.17 kflop.
+FLOP_PER_LINK_UPDATE = {"mixed": 9245, "fp64": 9513}
 # HBM-side bytes per k_flow_stage launch, averaged over the three RK stages,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this bench (fetch
 # scaled by 2.0 and writes by 1.0, the factors a 1 GiB streaming copy reads
-# back), with the schedules the autotuner picks; the run, its commands and the
-# per-stage figures are in PROFILE_FILE.  The mixed flow keeps its links as
-# two rows (12 reals), so it moves fewer bytes than the canonical 18-real
-# count of REALS_PER_LINK_UPDATE.
+# back), per schedule, since the schedule sets how often neighbour links come
+# from HBM rather than L2; the runs, their commands and the per-stage figures
+# are in the profile files.  The mixed flow keeps its links as two rows (12
+# reals), so it moves fewer bytes than the canonical 18-real count of
+# REALS_PER_LINK_UPDATE.  A schedule not profiled reports traffic null.
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
-PROFILED_TRAFFIC = {("mixed-32x64", "mixed"): (1.406e9, PROFILE_FILE), ("mixed-32x64", "fp64"): (6.372e9, PROFILE_FILE),
-                    ("mixed-48x96", "mixed"): (8.714e9, "profiles/k_flow_stage_48x48x48x96.md"),
-                    ("mixed-48x96", "fp64"): (25.65e9, "profiles/k_flow_stage_48x48x48x96.md")}
+PROFILE_FILE_48 = "profiles/k_flow_stage_48x48x48x96.md"
+PROFILED_TRAFFIC = {
+    ("mixed-32x64", "mixed", "column+lds+2row target=4096"): (1.441e9, PROFILE_FILE),
+    ("mixed-32x64", "mixed", "linear+lds+2row target=0"): (1.872e9, PROFILE_FILE),
+    ("mixed-32x64", "fp64", "column+lds target=4096"): (6.338e9, PROFILE_FILE),
+    ("mixed-48x96", "mixed", "column+lds+2row target=1024"): (8.736e9, PROFILE_FILE_48),
+    ("mixed-48x96", "fp64", "column+lds target=1024"): (25.54e9, PROFILE_FILE_48),
+}
+
+
+def schedule_label(tuner):
+    """'column+lds+2row target=4096' from an autotuner summary line."""
+    f = dict(kv.split("=", 1) for kv in tuner.split() if "=" in kv)
+    return f"{f.get('schedule')} target={f.get('target')}"
+
 
 WORKLOADS = {
     # 1000 sweeps from a Haar-random start put t0 (t^2 <E> = 0.3) near t = 5.1,
@@ -316,7 +329,8 @@ def main():
     bytes_step = 4 * V * REALS_PER_LINK_UPDATE * real
     # per launch, averaged over the three stage launches of a step
     achieved_gbs = bytes_step / (step_kernel_ms * 1e-3) / 1e9
-    traffic, traffic_source = PROFILED_TRAFFIC.get((args.workload, main_prec), (None, None))
+    traffic, traffic_source = PROFILED_TRAFFIC.get((args.workload, main_prec, schedule_label(r["tuner"])),
+                                                   (None, None))
     out = {
         "metric": METRIC, "value": lups, "unit": "link-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",

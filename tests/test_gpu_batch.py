@@ -132,6 +132,34 @@ def test_autotune_keeps_results(inputs, tmp_path):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_leaves_the_fields(inputs, precision):
+    """The tuner times flow steps on the batch's own fields and puts them
+    back: every lattice downloads the same bits before and after tuning, also
+    when the tuning starts from the field a flow left, and the flow after it
+    goes on as an untuned batch's does."""
+    with BatchContext(DIMS, precision) as a, BatchContext(DIMS, precision) as b:
+        for ctx in (a, b):
+            for c, L in enumerate(inputs):
+                ctx.upload(c, L)
+        before = [a.download(c) for c in range(len(DIMS))]
+        assert "measured" in a.autotune(None)
+        for c in range(len(DIMS)):
+            assert np.array_equal(a.download(c), before[c]), c
+        a.flow(0.02, 0.06, 1, use_graph=False)
+        b.flow(0.02, 0.06, 1, use_graph=False)
+        mid = [a.download(c) for c in range(len(DIMS))]
+        a.autotune(None)
+        for c in range(len(DIMS)):
+            assert np.array_equal(a.download(c), mid[c]), c
+            assert np.array_equal(b.download(c), mid[c]), c
+        got, want = a.flow(0.02, 0.06, 1, use_graph=True), b.flow(0.02, 0.06, 1, use_graph=False)
+        for c in range(len(DIMS)):
+            assert np.array_equal(a.download(c), b.download(c)), c
+        m = ~np.isnan(want[..., 1])
+        assert np.allclose(got[m][..., [0, 1, 3]], want[m][..., [0, 1, 3]], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("grid", [0, 8])
 @pytest.mark.parametrize("dims", [[(8, 8, 2, 2), (16, 4, 2, 2), (32, 2, 2, 4)],
                                   [(8, 8, 2, 2), (6, 4, 4, 2), (16, 4, 2, 2), (12, 2, 2, 2)]])

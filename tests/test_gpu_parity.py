@@ -238,6 +238,27 @@ def test_autotune_picks_a_schedule(therm8, tmp_path):
         assert np.array_equal(ctx.download(), links)  # tuning leaves the field alone
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_between_flow_calls(precision):
+    """The tuner times flow steps on the context's own field and puts it back:
+    tuned between two flow calls (the mixed field then in two-row form), the
+    links and the series go on exactly as without tuning."""
+    dims = (8, 8, 4, 8)
+    links = orc.hot_start(dims, 71)
+    with LatticeContext(dims, precision) as a, LatticeContext(dims, precision) as b:
+        for ctx in (a, b):
+            ctx.upload(links)
+            ctx.flow(0.02, 0.06, 1)
+        mid = b.download()
+        assert "measured" in a.autotune(None)
+        np.testing.assert_array_equal(a.download(), mid)
+        got, want = a.flow(0.02, 0.1, 2), b.flow(0.02, 0.1, 2)
+        np.testing.assert_array_equal(a.download(), b.download())
+        m = ~np.isnan(want[:, 1])
+        assert np.allclose(got[m][:, [0, 1, 3]], want[m][:, [0, 1, 3]], rtol=1e-14, atol=0)
+        assert np.allclose(got[m][:, 2], want[m][:, 2], rtol=0, atol=1e-13)
+
+
 @pytest.mark.parametrize("dims", [(6, 4, 4, 8), (12, 4, 2, 4)])
 def test_lds_variant_without_whole_rows(dims):
     """Lx not dividing 64: only own-site links come from LDS; results unchanged."""

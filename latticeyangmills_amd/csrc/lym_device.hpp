@@ -850,15 +850,24 @@ template <typename S, int AUX = 0> LYM_DI void store_h8(S *__restrict__ a, long 
     }
 }
 
+// 1/sqrt(n) in FP64.  STORED_FP32: the result feeds a link stored in FP32,
+// so the hardware estimate (v_rsq_f64, relative error 5.2e-8 measured on
+// gfx950) refined by one Newton step (4.1e-15) is ample, seven orders below
+// the storage rounding; otherwise rsqrt() and one more Newton step (2e-16).
+template <bool STORED_FP32> LYM_DI double rsqrt_refined(double n)
+{
+    const double r = STORED_FP32 ? __builtin_amdgcn_rsq(n) : rsqrt(n);
+    return r * fma(-0.5 * n, r * r, 1.5);
+}
+
 // Project a near-unitary matrix back onto SU(3): Gram-Schmidt on rows 0, 1,
-// row 2 = conj(row0 x row1) (so det = 1).
-LYM_DI void reunitarize(su3<double> &m)
+// row 2 = conj(row0 x row1) (so det = 1).  STORED_FP32 as rsqrt_refined.
+template <bool STORED_FP32 = false> LYM_DI void reunitarize(su3<double> &m)
 {
     double n0 = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; j++) n0 = fma(m.e[j].im, m.e[j].im, fma(m.e[j].re, m.e[j].re, n0));
-    double r0 = rsqrt(n0);
-    r0 = r0 * fma(-0.5 * n0, r0 * r0, 1.5);  // one Newton step for full FP64 accuracy
+    const double r0 = rsqrt_refined<STORED_FP32>(n0);
 #pragma unroll
     for (int j = 0; j < 3; j++) m.e[j] = cplx<double>{m.e[j].re * r0, m.e[j].im * r0};
     cplx<double> p{0.0, 0.0};  // <row0|row1>
@@ -870,8 +879,7 @@ LYM_DI void reunitarize(su3<double> &m)
     double n1 = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; j++) n1 = fma(m.e[3 + j].im, m.e[3 + j].im, fma(m.e[3 + j].re, m.e[3 + j].re, n1));
-    double r1 = rsqrt(n1);
-    r1 = r1 * fma(-0.5 * n1, r1 * r1, 1.5);
+    const double r1 = rsqrt_refined<STORED_FP32>(n1);
 #pragma unroll
     for (int j = 0; j < 3; j++) m.e[3 + j] = cplx<double>{m.e[3 + j].re * r1, m.e[3 + j].im * r1};
 #pragma unroll

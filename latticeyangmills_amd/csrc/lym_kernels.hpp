@@ -279,7 +279,7 @@ LYM_DI void flow_link(const Src &src, F2<S> *__restrict__ wout, S *__restrict__ 
     }
     su3<double> E = (sizeof(S) == 4) ? exp_iq<kShortSeriesTermsF32, kSeriesTermsF32>(q) : exp_iq(q);
     su3<double> W = mul_nn(E, convert<double>(U));
-    if constexpr (REUNIT) reunitarize(W);
+    if constexpr (REUNIT) reunitarize<sizeof(S) == 4>(W);
     constexpr int ROWS = Src::rows;  // the output field takes the input's form
     store_link_r<S, double, kStreamAux, ROWS>(wout + (long)MU * plane_elems<ROWS>() * V, V, unsigned(s), W);
 }

@@ -188,8 +188,10 @@ BATCH_DIMS = [(8, 8, 8, 8)] * 16 + [(12, 12, 12, 12)] * 16 + [(16, 16, 16, 16)] 
 def batched_ensemble(tune_cache, every=10, use_graph=True):
     """64 uneven lattices (16 each of 8^4, 12^4, 16^4, 16^3x32), beta = 6.0,
     Haar hot starts with seeds 0..63, 100 Metropolis sweeps, then the mixed
-    flow eps = 0.01 to t = 2.0 (199 steps, observables every `every` steps) as
-    one persistent launch per RK stage, graph-captured, autotuned; against the
+    flow eps = 0.01 to t = 2.0 (199 steps, observables every `every` steps),
+    graph-captured, in the launch form and schedule the autotuner picks (one
+    launch per RK stage, or one persistent launch per measurement interval);
+    both launch forms are also timed on their own; against the
     64 lattices flowed one after another by the plain FP64 single-lattice path,
     and (to separate the batching gain from the precision gain) by the mixed
     single-lattice path.  use_graph=False launches the batch eagerly (for a
@@ -212,6 +214,29 @@ def batched_ensemble(tune_cache, every=10, use_graph=True):
         t0 = time.perf_counter()
         series_b = b.flow(eps, tau, every, use_graph=use_graph)
         t_batch = time.perf_counter() - t0
+        # both launch forms of the tuned schedule, best of three graph replays
+        # each: one launch per RK stage, and one persistent launch per
+        # measurement interval (grid barriers between stages)
+        tuned = b.info()
+        forms = {}
+        for whole in (False, True):
+            b.set_schedule(0 if whole else tuned["grid"], tuned["kind"], tuned["target"], tuned["lds"],
+                           tuned["two_row"], whole_flow=whole)
+            best = None
+            for rep in range(4):
+                for c, L in enumerate(confs):
+                    b.upload(c, L)
+                t0 = time.perf_counter()
+                b.flow(eps, tau, every, use_graph=use_graph)
+                el = time.perf_counter() - t0
+                if rep:  # the first run captures the graph
+                    best = el if best is None else min(best, el)
+            forms["whole_flow" if whole else "per_stage"] = {
+                "configs_per_s": len(BATCH_DIMS) / best, "seconds": best, "grid": b.info()["grid"]}
+        forms["note"] = ("per_stage: three launches per flow step (597 for the flow), one workgroup per schedule slot "
+                         "unless the tuner chose a persistent grid; whole_flow: one persistent launch per 10-step "
+                         "measurement interval (20 launches), grid-wide barriers between RK stages, as many "
+                         "workgroups as the device holds at once")
     def sequential(precision):
         ctxs = []
         for c, L in enumerate(confs):
@@ -234,7 +259,7 @@ def batched_ensemble(tune_cache, every=10, use_graph=True):
     return {"configs_per_s": n / t_batch, "seconds": t_batch, "sequential_plain_fp64_configs_per_s": n / t_seq,
             "sequential_seconds": t_seq, "speedup": t_seq / t_batch, "tuner": tuner,
             "sequential_mixed_configs_per_s": n / t_seq_mixed, "speedup_vs_sequential_mixed": t_seq_mixed / t_batch,
-            "max_rel_err_E_vs_fp64": e_err, "graph": use_graph,
+            "max_rel_err_E_vs_fp64": e_err, "graph": use_graph, "launch_forms": forms,
             "workload": "64 lattices: 16 x 8^4, 16 x 12^4, 16 x 16^4, 16 x 16^3x32; beta 6.0; Haar seeds 0..63; "
                         f"100 sweeps; mixed flow eps {eps} to t {tau} (199 steps), observables every {every}"}
 

@@ -52,8 +52,24 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
  * lym_set_schedule (lattices stay whole, in order; LYM_SCHED_TWO_ROW for a
  * mixed batch keeps every lattice in two-row form while lym_batch_flow runs);
  * grid = 0 means one workgroup per schedule slot, otherwise a persistent grid
- * of that many workgroups (rounded up to a multiple of 8). */
+ * of that many workgroups (rounded up to a multiple of 8).
+ * With LYM_SCHED_WHOLE_FLOW or-ed into kind, lym_batch_flow runs the steps
+ * between two measurements (all of them when measure_every exceeds the step
+ * count) of every lattice as ONE persistent launch, with a grid-wide barrier
+ * between RK stages; grid = 0 then takes as many workgroups as the device
+ * holds at once (rounded down to a multiple of 8), and a larger grid is
+ * refused, since every workgroup must be resident for the barriers.  The
+ * barrier waits are bounded (lym_batch_set_barrier_timeout, 5 s): a launch
+ * whose workgroups cannot all be resident ends, and lym_batch_flow returns
+ * LYM_ERR_HIP.  The links equal
+ * those of the per-stage launches bit for bit. */
+#define LYM_SCHED_WHOLE_FLOW 64
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target);
+/* Bound of each grid-barrier wait of the whole-flow launch, in seconds
+ * (default 5; no counterpart in the reference, which has no device).  A wait
+ * past it ends the launch and lym_batch_flow returns LYM_ERR_HIP; 0 makes
+ * every wait give up at once, which exercises that path from the host. */
+int lym_batch_set_barrier_timeout(lym_batch *b, double seconds);
 /* Times the candidate shapes on the batch itself and keeps the fastest, as
  * lym_autotune (the default schedule unless another is faster by more than
  * 1.5 %; then persistent grids of the chosen order under the same rule);

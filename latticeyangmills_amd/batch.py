@@ -5,6 +5,8 @@ import numpy as np
 
 from .capi import CANDIDATES, LYM_OK, PRECISIONS, SCHED_LDS, SCHED_TWO_ROW, SCHEDULES, LymError, _D, _dptr, _links_ptrs, lib
 
+SCHED_WHOLE_FLOW = 64  # LYM_SCHED_WHOLE_FLOW
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _sigs_done = False
@@ -28,6 +30,7 @@ def _sigs(L):
     L.lym_batch_observables.argtypes = [_P, _D]
     L.lym_batch_flow.argtypes = [_P, c.c_double, c.c_double, _I, _I, _D, _I, c.POINTER(_I)]
     L.lym_batch_set_schedule.argtypes = [_P, _I, _I, _I]
+    L.lym_batch_set_barrier_timeout.argtypes = [_P, ctypes.c_double]
     L.lym_batch_autotune.argtypes = [_P, c.c_char_p, c.c_char_p, _I]
     _sigs_done = True
     return L
@@ -101,9 +104,13 @@ class BatchContext:
                                              series.shape[1], ctypes.byref(r)), "lym_batch_flow")
         return series[:, : r.value]
 
-    def set_schedule(self, grid=0, kind="column", target=4096, lds=False, two_row=False):
-        code = SCHEDULES[kind] | (SCHED_LDS if lds else 0) | (SCHED_TWO_ROW if two_row else 0)
+    def set_schedule(self, grid=0, kind="column", target=4096, lds=False, two_row=False, whole_flow=False):
+        code = (SCHEDULES[kind] | (SCHED_LDS if lds else 0) | (SCHED_TWO_ROW if two_row else 0)
+                | (SCHED_WHOLE_FLOW if whole_flow else 0))
         self._check(self._lib.lym_batch_set_schedule(self._h, grid, code, target), "lym_batch_set_schedule")
+
+    def set_barrier_timeout(self, seconds):
+        self._check(self._lib.lym_batch_set_barrier_timeout(self._h, seconds), "lym_batch_set_barrier_timeout")
 
     def autotune(self, cache_path=None):
         buf = ctypes.create_string_buffer(256)
@@ -117,4 +124,5 @@ class BatchContext:
                                              ctypes.byref(kind), ctypes.byref(target)), "lym_batch_info")
         return dict(nconf=n.value, total_sites=sites.value, grid=grid.value,
                     kind=[k for k, v in SCHEDULES.items() if v == kind.value & 3][0],
-                    lds=bool(kind.value & SCHED_LDS), two_row=bool(kind.value & SCHED_TWO_ROW), target=target.value)
+                    lds=bool(kind.value & SCHED_LDS), two_row=bool(kind.value & SCHED_TWO_ROW),
+                    whole_flow=bool(kind.value & SCHED_WHOLE_FLOW), target=target.value)

@@ -30,9 +30,16 @@ HIP_FLAGS = [
 # One object per translation unit, compiled in parallel: the stage and
 # Metropolis kernel families are instantiated per link precision in units of
 # their own (lym_launch.hpp), the C ABI of the two paths in lym_capi / lym_batch.
-LIB_SOURCES = ["lym_bstage_f64.hip", "lym_bstage_f32.hip", "lym_bstage_f32r2.hip", "lym_stage_f64.hip",
-               "lym_stage_f32.hip", "lym_stage_f32r2.hip", "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip",
-               "lym_batch.hip", "lym_noalloc.hip"]
+LIB_SOURCES = ["lym_bflow_f64.hip", "lym_bflow_f32.hip", "lym_bflow_f32r2.hip", "lym_bstage_f64.hip",
+               "lym_bstage_f32.hip", "lym_bstage_f32r2.hip", "lym_stage_f64.hip", "lym_stage_f32.hip",
+               "lym_stage_f32r2.hip", "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip", "lym_batch.hip",
+               "lym_noalloc.hip"]
+# The whole-flow kernels loop over RK stages and schedule slots; machine-level
+# loop-invariant code motion would hoist the constants and lane offsets of the
+# stage body out of those loops and hold them in registers across them (about
+# 155 VGPRs, 3 waves per SIMD, against 96 and 5 for the one-launch-per-stage
+# kernels), so their units are built without it.
+UNIT_FLAGS = {s: ["-mllvm", "-disable-machine-licm"] for s in LIB_SOURCES if s.startswith("lym_bflow_")}
 OBJ_DIR = os.path.join(PKG, "_build")
 
 
@@ -52,8 +59,8 @@ def build_lym(extra=()):
     objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in LIB_SOURCES]
 
     def compile_one(i):
-        _run([HIPCC, *HIP_FLAGS, *extra, "-c", "-I", INCLUDE, "-I", CSRC, "-o", objs[i],
-              os.path.join(CSRC, LIB_SOURCES[i])])
+        _run([HIPCC, *HIP_FLAGS, *UNIT_FLAGS.get(LIB_SOURCES[i], []), *extra, "-c", "-I", INCLUDE, "-I", CSRC,
+              "-o", objs[i], os.path.join(CSRC, LIB_SOURCES[i])])
 
     with concurrent.futures.ThreadPoolExecutor(_jobs()) as pool:
         for f in [pool.submit(compile_one, i) for i in range(len(LIB_SOURCES))]:

@@ -1,9 +1,11 @@
-// Batched ensemble path: many independent lattices of uneven sizes flowed by
-// one launch per RK stage for the whole batch, the whole flow loop captured in
-// a HIP graph, the launch shape picked by an autotuner (declared in
-// include/lym_batch.h; kernels in lym_batch_kernels.hpp).  The grid is either
-// one workgroup per schedule slot or a persistent grid (a multiple of 8, so a
-// workgroup keeps its XCD) that strides over the slots.
+// Batched ensemble path: many independent lattices of uneven sizes flowed
+// together, the whole flow loop captured in a HIP graph, the launch shape
+// picked by an autotuner (declared in include/lym_batch.h; kernels in
+// lym_batch_kernels.hpp).  Three launch shapes: one launch per RK stage for
+// the whole batch with one workgroup per schedule slot, the same with a
+// persistent grid (a multiple of 8, so a workgroup keeps its XCD) striding
+// over the slots, or one persistent launch for all the steps between two
+// measurements, with grid-wide barriers between RK stages (SCHED_WHOLE_FLOW).
 #include <algorithm>
 #include <array>
 #include <chrono>
@@ -49,6 +51,8 @@ struct lym_batch {
     lym::HostSchedule sched;
     int *d_order = nullptr;
     std::string tuned;
+    lym::FlowSync *sync = nullptr;        // barrier state of the whole-flow launch
+    unsigned long long sync_limit = 0;    // its bounded wait, in wall-clock ticks (5 s)
     hipGraphExec_t graph = nullptr;
     std::string graph_key;
     int graph_parity_flip = 0;
@@ -94,29 +98,12 @@ void drop_graph(lym_batch *b)
     b->graph_key.clear();
 }
 
-int apply_schedule(lym_batch *b, int grid, int kind, int target)
+// LDS staging mode of the batch stage kernel under schedule kind `kind`: slab
+// when every lattice fits it, per lattice (4) when only some do, else whole
+// x-rows or own links.
+int batch_lds_mode(const lym_batch *b, int kind)
 {
-    std::vector<long> run0;
-    for (auto &d : b->desc) run0.push_back(d.tile0);
-    lym::HostSchedule h = lym::build_schedule(b->dims, run0, kind, target);
-    if (b->d_order) (void)hipFree(b->d_order);
-    b->d_order = nullptr;
-    if (!h.order.empty()) {
-        BHIP(b, hipMalloc(&b->d_order, sizeof(int) * h.order.size()));
-        BHIP(b, hipMemcpy(b->d_order, h.order.data(), sizeof(int) * h.order.size(), hipMemcpyHostToDevice));
-    }
-    b->sched = std::move(h);
-    b->grid = grid > 0 ? ((grid + lym::kXcds - 1) / lym::kXcds) * lym::kXcds : 0;
-    b->rows = (b->sched.kind & lym::SCHED_TWO_ROW) ? 2 : 3;
-    drop_graph(b);
-    return LYM_OK;
-}
-
-// LDS staging mode of the batch stage kernel: slab when every lattice fits
-// it, per lattice (4) when only some do, else whole x-rows or own links.
-int batch_lds_mode(const lym_batch *b)
-{
-    if (!(b->sched.kind & lym::SCHED_LDS)) return 0;
+    if (!(kind & lym::SCHED_LDS)) return 0;
     bool xrows = true, slab = true, any_slab = false;
     for (auto &d : b->dims) {
         xrows = xrows && (lym::kSitesPerBlock % d[0] == 0);
@@ -124,6 +111,58 @@ int batch_lds_mode(const lym_batch *b)
         any_slab = any_slab || lym::slab_fits(d.data());
     }
     return slab ? 3 : any_slab ? 4 : xrows ? 2 : 1;
+}
+int batch_lds_mode(const lym_batch *b) { return batch_lds_mode(b, b->sched.kind); }
+
+// Workgroups of the whole-flow launch under schedule kind `kind`: `want`
+// rounded up to a multiple of 8, or with want = 0 as many as the device holds
+// at once (the kernel's workgroups per CU, from its registers and LDS, times
+// the CUs) rounded down to a multiple of 8; never more than that, since every
+// workgroup must be resident for the grid barriers, nor more than the
+// schedule has slots (rounded up to a multiple of 8).
+int whole_flow_grid(lym_batch *b, int kind, long slots, int want, int *grid)
+{
+    const int lds = batch_lds_mode(b, kind);
+    int per_cu = 0, cus = 0;
+    const hipError_t e = b->prec == LYM_PREC_FP64     ? lym::batch_flow_persistent_per_cu<double, 3>(lds, &per_cu)
+                         : (kind & lym::SCHED_TWO_ROW) ? lym::batch_flow_persistent_per_cu<float, 2>(lds, &per_cu)
+                                                       : lym::batch_flow_persistent_per_cu<float, 3>(lds, &per_cu);
+    BHIP(b, e);
+    BHIP(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    const long resident = long(per_cu) * cus / lym::kXcds * lym::kXcds;
+    if (resident < lym::kXcds) return bfail(b, LYM_ERR_HIP, "the whole-flow kernel does not fit the device");
+    const long g = want > 0 ? (long(want) + lym::kXcds - 1) / lym::kXcds * lym::kXcds : resident;
+    if (g > resident)
+        return bfail(b, LYM_ERR_ARG,
+                     "whole-flow grid of " + std::to_string(g) + " workgroups: the device holds " +
+                         std::to_string(resident) + " at once, and every one must be resident for the grid barriers");
+    *grid = int(std::min(g, (slots + lym::kXcds - 1) / lym::kXcds * lym::kXcds));
+    return LYM_OK;
+}
+
+int apply_schedule(lym_batch *b, int grid, int kind, int target)
+{
+    std::vector<long> run0;
+    for (auto &d : b->desc) run0.push_back(d.tile0);
+    lym::HostSchedule h = lym::build_schedule(b->dims, run0, kind, target);
+    int whole_grid = 0;
+    if (kind & lym::SCHED_WHOLE_FLOW) {
+        const long slots = h.order.empty() ? b->ntiles : long(lym::kXcds) * h.per_xcd;
+        if (int rc = whole_flow_grid(b, kind, slots, grid, &whole_grid)) return rc;
+    }
+    if (b->d_order) (void)hipFree(b->d_order);
+    b->d_order = nullptr;
+    if (!h.order.empty()) {
+        BHIP(b, hipMalloc(&b->d_order, sizeof(int) * h.order.size()));
+        BHIP(b, hipMemcpy(b->d_order, h.order.data(), sizeof(int) * h.order.size(), hipMemcpyHostToDevice));
+    }
+    b->sched = std::move(h);
+    b->grid = (kind & lym::SCHED_WHOLE_FLOW) ? whole_grid
+              : grid > 0                     ? ((grid + lym::kXcds - 1) / lym::kXcds) * lym::kXcds
+                                             : 0;
+    b->rows = (b->sched.kind & lym::SCHED_TWO_ROW) ? 2 : 3;
+    drop_graph(b);
+    return LYM_OK;
 }
 
 template <typename S, int STAGE>
@@ -175,9 +214,66 @@ template <typename S> int enqueue_step(lym_batch *b, double eps)
     b->cur = 1 - b->cur;
     return LYM_OK;
 }
+// nsteps steps as one whole-flow launch (SCHED_WHOLE_FLOW), its arrival
+// count zeroed by a kernel just before it (graph capture records no memset).
+template <typename S> int enqueue_whole(lym_batch *b, double eps, int nsteps)
+{
+    using F2 = lym::F2<S>;
+    F2 *x = static_cast<F2 *>(b->W[b->cur]);
+    F2 *y = static_cast<F2 *>(b->W[1 - b->cur]);
+    S *acc = static_cast<S *>(b->acc);
+    lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 0);
+    const int lds = batch_lds_mode(b);
+    const unsigned grid = unsigned(b->grid);
+    if (sizeof(S) == 4 && b->form == 2)
+        lym::launch_batch_flow_persistent<float, 2>(lds, grid, b->stream, reinterpret_cast<float2 *>(x),
+                                                    reinterpret_cast<float2 *>(y), reinterpret_cast<float *>(acc),
+                                                    b->d_desc, b->nconf, b->ntiles, dev_sched(b), eps, nsteps, b->sync,
+                                                    b->sync_limit);
+    else
+        lym::launch_batch_flow_persistent<S, 3>(lds, grid, b->stream, x, y, acc, b->d_desc, b->nconf, b->ntiles,
+                                                dev_sched(b), eps, nsteps, b->sync, b->sync_limit);
+    BLAUNCHED(b, "k_batch_flow_persistent");
+    if (nsteps & 1) b->cur = 1 - b->cur;
+    return LYM_OK;
+}
+// Steps per whole-flow launch at most, so that the arrival count (3 per step
+// per workgroup) and the slot counters (3 per step times the XCD's slots and
+// workgroups) stay below 2^32.
+int whole_max_steps(const lym_batch *b)
+{
+    const long slots = b->sched.order.empty() ? b->ntiles : long(lym::kXcds) * b->sched.per_xcd;
+    const long per_xcd = (slots + lym::kXcds - 1) / lym::kXcds;
+    const long per_step = 3L * std::max<long>(b->grid, per_xcd + b->grid / lym::kXcds);
+    return int(std::max<long>(1, std::min<long>(1L << 20, 0xFFFFFFFFL / per_step - 1)));
+}
+
+int whole(lym_batch *b, double eps, int nsteps)
+{
+    for (int done = 0, rc; done < nsteps; done += whole_max_steps(b)) {
+        const int n = std::min(nsteps - done, whole_max_steps(b));
+        if ((rc = b->prec == LYM_PREC_FP64 ? enqueue_whole<double>(b, eps, n) : enqueue_whole<float>(b, eps, n)))
+            return rc;
+    }
+    return LYM_OK;
+}
+
 int step(lym_batch *b, double eps)
 {
+    if (b->sched.kind & lym::SCHED_WHOLE_FLOW) return whole(b, eps, 1);
     return b->prec == LYM_PREC_FP64 ? enqueue_step<double>(b, eps) : enqueue_step<float>(b, eps);
+}
+
+// Whether a whole-flow launch since the flag was last cleared gave up at a
+// grid barrier (its workgroups were not all resident): the links are then
+// not a flow result.  Waits for the stream.
+int whole_flow_aborted(lym_batch *b, bool *aborted)
+{
+    int flag = 0;
+    BHIP(b, hipMemcpyAsync(&flag, &b->sync->abort, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    *aborted = flag != 0;
+    return LYM_OK;
 }
 
 template <typename S> int enqueue_obs(lym_batch *b, int slot)
@@ -220,11 +316,23 @@ void normalise(const lym_batch *b, int c, const double *raw, double out[4])
     out[3] = raw[3] / qn;
 }
 
-// enqueue the whole flow (rows-1 steps, measurements into slots)
+// enqueue the whole flow (rows-1 steps, measurements into slots after the
+// steps t with t % every == 0); with SCHED_WHOLE_FLOW the steps between two
+// measurements run as one launch
 int enqueue_flow(lym_batch *b, double eps, int rows, int every)
 {
     int slot = 0, rc;
     if ((rc = to_form(b, b->rows)) || (rc = observables(b, slot++))) return rc;
+    if (b->sched.kind & lym::SCHED_WHOLE_FLOW) {
+        lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 1);
+        for (int t = 1; t < rows;) {
+            const int last = std::min((t + every - 1) / every * every, rows - 1);  // last step of the segment
+            if ((rc = whole(b, eps, last - t + 1))) return rc;
+            if (last % every == 0 && (rc = observables(b, slot++))) return rc;
+            t = last + 1;
+        }
+        return to_form(b, 3);
+    }
     for (int t = 1; t < rows; t++) {
         if ((rc = step(b, eps))) return rc;
         if (t % every == 0)
@@ -308,6 +416,14 @@ int lym_batch_create(lym_batch **out, int nconf, const int *dims, int precision,
     if (hipMalloc(&b->staging, sizeof(double) * 18 * size_t(maxV)) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&b->accepted, sizeof(unsigned long long) * nconf) != hipSuccess) return bail(LYM_ERR_NOMEM);
     if (hipMalloc(&b->d_desc, sizeof(ConfDesc) * nconf) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMalloc(&b->sync, sizeof(lym::FlowSync)) != hipSuccess) return bail(LYM_ERR_NOMEM);
+    if (hipMemset(b->sync, 0, sizeof(lym::FlowSync)) != hipSuccess) return bail(LYM_ERR_HIP);
+    {
+        int khz = 0;  // the constant wall clock of the barrier waits
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+            return bail(LYM_ERR_HIP);
+        b->sync_limit = 5000ull * (unsigned long long)khz;
+    }
     if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(ConfDesc) * nconf, hipMemcpyHostToDevice) != hipSuccess)
         return bail(LYM_ERR_HIP);
     if (ensure_slots(b, 4) != LYM_OK) return bail(LYM_ERR_NOMEM);
@@ -338,7 +454,7 @@ int lym_batch_destroy(lym_batch *b)
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->bounce.release();
     for (void *p : {b->W[0], b->W[1], b->acc, (void *)b->partials, (void *)b->obs, (void *)b->staging,
-                    (void *)b->accepted, (void *)b->d_desc, (void *)b->d_order})
+                    (void *)b->accepted, (void *)b->d_desc, (void *)b->d_order, (void *)b->sync})
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -528,7 +644,8 @@ int lym_batch_observables(lym_batch *b, double *out)
 int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target)
 {
     const int order = kind & 3;
-    if (!b || grid < 0 || (kind & ~(lym::SCHED_LDS | lym::SCHED_TWO_ROW | 3)) || order > lym::SCHED_COLUMN ||
+    if (!b || grid < 0 || (kind & ~(lym::SCHED_LDS | lym::SCHED_TWO_ROW | lym::SCHED_WHOLE_FLOW | 3)) ||
+        order > lym::SCHED_COLUMN ||
         (order == lym::SCHED_COLUMN && target < 1))
         return LYM_ERR_ARG;
     if ((kind & lym::SCHED_TWO_ROW) && b->prec != LYM_PREC_MIXED)
@@ -536,6 +653,16 @@ int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target)
     BHIP(b, hipSetDevice(b->device));
     BHIP(b, hipStreamSynchronize(b->stream));
     return apply_schedule(b, grid, kind, target);
+}
+
+int lym_batch_set_barrier_timeout(lym_batch *b, double seconds)
+{
+    if (!b || !(seconds >= 0) || seconds > 3600) return b ? bfail(b, LYM_ERR_ARG, "timeout outside [0, 3600] s") : LYM_ERR_ARG;
+    int khz = 0;
+    BHIP(b, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    b->sync_limit = (unsigned long long)(seconds * 1000.0 * khz);
+    drop_graph(b);  // the graph holds the old bound
+    return LYM_OK;
 }
 
 int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int use_graph, double *series,
@@ -587,6 +714,14 @@ int lym_batch_flow(lym_batch *b, double eps, double tau, int measure_every, int 
     std::vector<double> raw(size_t(4) * b->nconf * nmeas);
     BHIP(b, hipMemcpyAsync(raw.data(), b->obs, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
+    if (b->sched.kind & lym::SCHED_WHOLE_FLOW) {
+        bool aborted = false;
+        if ((rc = whole_flow_aborted(b, &aborted))) return rc;
+        if (aborted)
+            return bfail(b, LYM_ERR_HIP,
+                         "whole-flow launch: a workgroup waited past the limit at a grid barrier (the grid was not "
+                         "resident at once); the links are not a flow result");
+    }
     for (int c = 0; c < b->nconf; c++) {
         double tcur = 0.0;
         int slot = 0;
@@ -698,6 +833,32 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
             rc = lym::tune_rounds(4, 3, time_grid, &best, &best_ms);
             best_grid = grids[best];
         }
+        if (!rc) {
+            // ... and that shape against one whole-flow launch per flow segment
+            // (timed over one step, i.e. three stages and two grid barriers);
+            // a whole-flow launch that gave up waiting at a barrier, or a
+            // kernel that does not fit the device, does not count
+            const int kinds[2] = {best_kind, best_kind | lym::SCHED_WHOLE_FLOW}, wgrids[2] = {best_grid, 0};
+            auto time_whole = [&](int i, float *ms) -> int {
+                if (int rc = apply_schedule(b, wgrids[i], kinds[i], best_target)) {
+                    if (i == 0) return rc;
+                    *ms = 1e30f;
+                    return apply_schedule(b, wgrids[0], kinds[0], best_target);
+                }
+                if (i == 1) lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 1);
+                if (int rc = time_step(ms)) return rc;
+                bool aborted = false;
+                if (i == 1)
+                    if (int rc = whole_flow_aborted(b, &aborted)) return rc;
+                if (aborted) *ms = 1e30f;
+                return LYM_OK;
+            };
+            rc = lym::tune_rounds(2, 3, time_whole, &best, &best_ms);
+            if (!rc) {
+                best_kind = kinds[best];
+                best_grid = wgrids[best];
+            }
+        }
         // the saved fields back, whatever the outcome
         const hipError_t e = hipMemcpyAsync(b->W[b->cur], save.p, bytes, hipMemcpyDeviceToDevice, b->stream);
         b->form = 3;
@@ -713,8 +874,10 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
     int rc = apply_schedule(b, best_grid, best_kind, best_target);
     if (rc) return rc;
     if (summary && summary_len > 0)
-        snprintf(summary, size_t(summary_len), "grid=%d schedule=%s target=%d step_ms=%.4f source=%s tiles=%ld",
-                 best_grid, lym::schedule_name(best_kind), best_target, best_ms, b->tuned.c_str(), b->ntiles);
+        snprintf(summary, size_t(summary_len), "grid=%d schedule=%s%s target=%d step_ms=%.4f source=%s tiles=%ld",
+                 b->grid, lym::schedule_name(best_kind & ~lym::SCHED_WHOLE_FLOW),
+                 (best_kind & lym::SCHED_WHOLE_FLOW) ? "+whole" : "", best_target, best_ms, b->tuned.c_str(),
+                 b->ntiles);
     return LYM_OK;
 }
 

@@ -49,7 +49,8 @@ template <int ROWS> LYM_DI long field_off(const ConfDesc &d) { return ROWS == 2 
 
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS>
 LYM_DI void batch_slot(long k, const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
-                       const ConfDesc *__restrict__ desc, int nconf, Sched sc, double eps, F2<S> *lds)
+                       const ConfDesc *__restrict__ desc, int nconf, Sched sc, double eps, F2<S> *lds,
+                       unsigned tid = threadIdx.x)
 {
     const long tile = sched_run(sc, unsigned(k));  // uniform over the workgroup
     if (tile < 0) return;
@@ -58,11 +59,11 @@ LYM_DI void batch_slot(long k, const F2<S> *__restrict__ win, F2<S> *__restrict_
     const long site0 = (tile - d.tile0) * kSitesPerBlock, off = field_off<ROWS>(d);
     if constexpr (LDS == 4) {
         if (d.slab)
-            flow_block<S, STAGE, REUNIT, 3, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+            flow_block<S, STAGE, REUNIT, 3, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds, tid);
         else
-            flow_block<S, STAGE, REUNIT, 1, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+            flow_block<S, STAGE, REUNIT, 1, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds, tid);
     } else {
-        flow_block<S, STAGE, REUNIT, LDS, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds);
+        flow_block<S, STAGE, REUNIT, LDS, ROWS>(win + off, wout + off, acc + d.acc_off, d.g, site0, eps, lds, tid);
     }
 }
 
@@ -84,6 +85,159 @@ void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
     } else {
         if (long(blockIdx.x) < slots)
             batch_slot<S, STAGE, REUNIT, LDS, ROWS>(blockIdx.x, win, wout, acc, desc, nconf, sc, eps, lds);
+    }
+}
+
+// ------------------------------------------- persistent whole-flow launch ---
+
+// State of a whole-flow launch, in device memory: the arrival count of its
+// grid barriers and one slot counter per XCD (both zeroed by
+// k_flow_sync_reset before each launch; the counters on lines of their own),
+// and a flag raised by a workgroup that gave up waiting (cleared once per
+// flow call).
+struct FlowSync {
+    unsigned arrivals;
+    int abort;
+    unsigned pad[62];
+    unsigned next[kXcds * 64];  // next[64 x]: slots of XCD x handed out so far in this launch
+};
+
+static __global__ void k_flow_sync_reset(FlowSync *s, int clear_abort)
+{
+    s->arrivals = 0u;
+    for (int x = 0; x < kXcds; x++) s->next[64 * x] = 0u;
+    if (clear_abort) s->abort = 0;
+}
+
+// Whether no workgroup has given up, the same answer for the whole workgroup.
+LYM_DI bool flow_sync_ok(FlowSync *sync)
+{
+    __shared__ int ok;
+    if (threadIdx.x == 0) ok = !__hip_atomic_load(&sync->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const bool r = ok != 0;
+    __syncthreads();
+    return r;
+}
+
+// Grid-wide barrier of the whole-flow launch.  Every workgroup of the launch
+// is resident at once (the host sizes the grid from the kernel's occupancy),
+// so each may wait for all the others: the n-th barrier of a launch waits
+// until the arrival count reaches n * gridDim.x.  The workgroup's barrier
+// orders its waves' stores before thread 0's agent-scope release fence and
+// arrival; after the count is reached thread 0's acquire fence, then the
+// workgroup's barrier, order them before every wave's next reads, so the next
+// RK stage reads the links the other workgroups (on every XCD) wrote.  (Fences
+// in every wave instead of one per workgroup cost the 64-lattice ensemble
+// 17 %.)  The wait is bounded: past `limit` ticks of the constant-rate wall
+// clock, after kMaxSyncPolls polls, or as soon as another workgroup has given
+// up, it raises the abort flag and returns false for the whole workgroup; the
+// caller then returns, so the launch always ends.
+constexpr unsigned kMaxSyncPolls = 1u << 26;
+LYM_DI bool grid_sync(FlowSync *sync, unsigned target, unsigned long long limit)
+{
+    __shared__ int go;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(&sync->arrivals, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long t0 = (unsigned long long)wall_clock64();
+        int ok = 1;
+        unsigned polls = 0;
+        while (__hip_atomic_load(&sync->arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&sync->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                (unsigned long long)wall_clock64() - t0 > limit || ++polls > kMaxSyncPolls) {
+                __hip_atomic_store(&sync->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        __threadfence();
+        go = ok;
+    }
+    __syncthreads();
+    return go != 0;
+}
+
+// Waves per SIMD the registers of the whole-flow kernel must allow: four for
+// the FP32 slab modes (3, 4), which then need no spills (held to five, as
+// the one-slot stage kernel of mode 3 is, the loops' extra live values spill
+// 14 VGPRs in two-row form, 50 in three-row form, and the ensemble flow runs
+// 2-3 % slower); the others take what they need.
+template <typename S, int LDS> constexpr int persist_min_waves()
+{
+    return sizeof(S) == 4 && (LDS == 3 || LDS == 4) ? 4 : 1;
+}
+
+// One RK stage of the whole-flow launch: the workgroups on XCD x take the
+// schedule slots of that XCD (slot 8 e + x for entry e of its row, the slots
+// the one-launch-per-stage kernels give that XCD) one at a time from the
+// XCD's counter, so that a workgroup that finishes early takes more, as the
+// hardware dispatcher does between launches.  Every workgroup of the XCD
+// takes one value past the row per stage, so stage j of a launch starts its
+// counter at j (per_xcd + gridDim.x / 8).  Thread 0 takes the next value
+// while the workgroup works on the current slot (two LDS words alternate, so
+// one workgroup barrier per slot suffices).
+template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS>
+LYM_DI void whole_flow_stage(unsigned j, const F2<S> *win, F2<S> *wout, S *__restrict__ acc,
+                             const ConfDesc *__restrict__ desc, int nconf, long slots, long per_xcd, Sched sc,
+                             double eps, F2<S> *lds, FlowSync *sync, unsigned tid)
+{
+    __shared__ unsigned taken[2];
+    const unsigned x = blockIdx.x % kXcds;
+    unsigned *counter = &sync->next[64 * x];
+    const unsigned base = j * unsigned(per_xcd + gridDim.x / kXcds);
+    unsigned pending = 0;
+    if (threadIdx.x == 0) pending = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = 0;; i ^= 1) {
+        if (threadIdx.x == 0) taken[i] = pending;
+        __syncthreads();
+        const long e = long(taken[i]) - long(base);
+        if (e >= per_xcd) break;
+        if (threadIdx.x == 0)
+            pending = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const long k = kXcds * e + x;
+        if (k < slots) batch_slot<S, STAGE, REUNIT, LDS, ROWS>(k, win, wout, acc, desc, nconf, sc, eps, lds, tid);
+    }
+}
+
+// nsteps whole RK3 steps of every lattice of the batch in one launch: the
+// three stages of each step as in k_batch_flow_stage, with the slots handed
+// out by whole_flow_stage and a grid-wide barrier between consecutive stages;
+// fields alternate w0 -> w1 -> w0 -> w1 within a step and swap roles after
+// it (the result of nsteps steps is in w0 for nsteps even, w1 for odd).  Each
+// link is computed by the same code as in the per-stage launches, so the two
+// give the same bits whichever workgroup takes a slot.  gridDim.x is a
+// multiple of 8.
+template <typename S, bool REUNIT, int LDS, int ROWS = 3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(persist_min_waves<S, LDS>())))
+void k_batch_flow_persistent(F2<S> *w0, F2<S> *w1, S *__restrict__ acc, const ConfDesc *__restrict__ desc, int nconf,
+                             long ntiles, Sched sc, double eps, int nsteps, FlowSync *sync, unsigned long long limit)
+{
+    __shared__ F2<S> lds[stage_lds_elems<LDS == 4 ? 3 : LDS>()];
+    const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
+    const long per_xcd = (slots + kXcds - 1) / kXcds;
+    if (!flow_sync_ok(sync)) return;  // an earlier launch of this flow gave up
+    F2<S> *x = w0, *y = w1;
+    unsigned n = 0;  // stages done = grid barriers passed
+    // the thread index through an opaque copy per stage: what the stage body
+    // derives from it is formed there instead of being held across the loops
+    auto tid = []() {
+        unsigned t = threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return t;
+    };
+    for (int t = 0; t < nsteps; t++) {
+        whole_flow_stage<S, 1, REUNIT, LDS, ROWS>(n, x, y, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
+        if (!grid_sync(sync, ++n * gridDim.x, limit)) return;
+        whole_flow_stage<S, 2, REUNIT, LDS, ROWS>(n, y, x, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
+        if (!grid_sync(sync, ++n * gridDim.x, limit)) return;
+        whole_flow_stage<S, 3, REUNIT, LDS, ROWS>(n, x, y, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
+        if (t + 1 < nsteps && !grid_sync(sync, ++n * gridDim.x, limit)) return;
+        F2<S> *const z = x;
+        x = y;
+        y = z;
     }
 }
 

@@ -307,12 +307,15 @@ template <int LDS> constexpr int stage_lds_elems()
 // registers of five waves; the others take what they need.
 template <typename S, int LDS> constexpr int stage_min_waves() { return sizeof(S) == 4 && LDS == 3 ? 5 : 1; }
 
+// tid: the thread's index in the workgroup (threadIdx.x; a persistent caller
+// passes it through an opaque copy, so that values derived from it are formed
+// per run and not held in registers across its loop).
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3>
 LYM_DI void flow_block(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,
-                       const Geom g, long site0, double eps, F2<S> *__restrict__ lds)
+                       const Geom g, long site0, double eps, F2<S> *__restrict__ lds, unsigned tid = threadIdx.x)
 {
-    const int mu = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));  // uniform over the wave
-    const int lane = threadIdx.x & 63;
+    const int mu = __builtin_amdgcn_readfirstlane(int(tid >> 6));  // uniform over the wave
+    const int lane = tid & 63;
     if constexpr (LDS == 5) {
         // Tiles cover the lattice exactly (checked on the host), so every lane
         // is a site.  Tile r = tx + ntx (ty + nty zt), zt = z + Lz t: the same

@@ -87,6 +87,39 @@ void launch_batch_stage_r2(int lds, unsigned grid, hipStream_t st, const F2<floa
     launch_batch_stage_rows<float, STAGE, 2>(lds, grid, st, win, wout, acc, desc, nconf, ntiles, sc, eps);
 }
 
+// The whole-flow persistent launch over a batch (k_batch_flow_persistent):
+// nsteps RK3 steps of every lattice by grid workgroups that are all resident
+// at once; lds as launch_batch_stage, ROWS the form of the fields.
+template <typename S, int ROWS>
+void launch_batch_flow_persistent(int lds, unsigned grid, hipStream_t st, F2<S> *w0, F2<S> *w1, S *acc,
+                                  const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps, int nsteps,
+                                  FlowSync *sync, unsigned long long limit)
+{
+    constexpr bool re = sizeof(S) == 4;  // mixed path: reunitarise before the store
+#define LYM_PERSIST_CASE(L)                                                                                   \
+    k_batch_flow_persistent<S, re, L, ROWS><<<grid, 256, 0, st>>>(w0, w1, acc, desc, nconf, ntiles, sc, eps, \
+                                                                   nsteps, sync, limit)
+    switch (lds) {
+    case 0: LYM_PERSIST_CASE(0); break;
+    case 1: LYM_PERSIST_CASE(1); break;
+    case 2: LYM_PERSIST_CASE(2); break;
+    case 3: LYM_PERSIST_CASE(3); break;
+    default: LYM_PERSIST_CASE(4); break;
+    }
+#undef LYM_PERSIST_CASE
+}
+// Workgroups of that kernel one CU holds at once (its registers and LDS).
+template <typename S, int ROWS> hipError_t batch_flow_persistent_per_cu(int lds, int *per_cu)
+{
+    constexpr bool re = sizeof(S) == 4;
+    switch (lds) {
+    case 0: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_flow_persistent<S, re, 0, ROWS>, 256, 0);
+    case 1: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_flow_persistent<S, re, 1, ROWS>, 256, 0);
+    case 2: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_flow_persistent<S, re, 2, ROWS>, 256, 0);
+    case 3: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_flow_persistent<S, re, 3, ROWS>, 256, 0);
+    default: return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_batch_flow_persistent<S, re, 4, ROWS>, 256, 0);
+    }
+}
 // Candidate kind of a Metropolis launch: 0 = R S T product, 1 = exp(i eps H)
 // with the general exponential, 2 = exp(i eps H) with 8 eps^2 <= kSeriesC1Max
 // (series-only exponential).
@@ -159,6 +192,17 @@ extern template void launch_batch_stage_r2<2>(int, unsigned, hipStream_t, const 
                                               const ConfDesc *, int, long, Sched, double);
 extern template void launch_batch_stage_r2<3>(int, unsigned, hipStream_t, const F2<float> *, F2<float> *, float *,
                                               const ConfDesc *, int, long, Sched, double);
+#endif
+#ifndef LYM_INSTANTIATE_BFLOW
+#define LYM_BFLOW_EXTERN(S, ROWS)                                                                                   \
+    extern template void launch_batch_flow_persistent<S, ROWS>(int, unsigned, hipStream_t, F2<S> *, F2<S> *, S *,   \
+                                                               const ConfDesc *, int, long, Sched, double, int,     \
+                                                               FlowSync *, unsigned long long);                     \
+    extern template hipError_t batch_flow_persistent_per_cu<S, ROWS>(int, int *);
+LYM_BFLOW_EXTERN(double, 3)
+LYM_BFLOW_EXTERN(float, 3)
+LYM_BFLOW_EXTERN(float, 2)
+#undef LYM_BFLOW_EXTERN
 #endif
 #ifndef LYM_INSTANTIATE_METRO
 LYM_METRO_EXTERN(double)

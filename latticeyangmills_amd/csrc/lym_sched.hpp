@@ -30,6 +30,9 @@ constexpr int SCHED_LDS = 16;
 // flag or-ed into a kind (mixed precision, single lattice): the flow keeps the
 // links in two-row form while it runs
 constexpr int SCHED_TWO_ROW = 32;
+// flag or-ed into a batch kind: the flow steps between two measurements run
+// as one persistent launch with grid-wide barriers between RK stages
+constexpr int SCHED_WHOLE_FLOW = 64;
 constexpr int kXcds = 8;
 constexpr int kRunSites = 64;
 
@@ -96,7 +99,7 @@ inline HostSchedule build_schedule(const std::vector<std::array<int, 4>> &dims, 
     HostSchedule h;
     h.kind = kind;
     h.target = target;
-    kind &= ~(SCHED_LDS | SCHED_TWO_ROW);
+    kind &= ~(SCHED_LDS | SCHED_TWO_ROW | SCHED_WHOLE_FLOW);
     if (kind == SCHED_LINEAR) return h;
     std::vector<int> all;
     for (size_t c = 0; c < dims.size(); c++) {

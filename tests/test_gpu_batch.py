@@ -10,6 +10,7 @@ import pytest
 
 from latticeyangmills_amd import LatticeContext
 from latticeyangmills_amd.batch import BatchContext
+from latticeyangmills_amd.capi import LymError
 from tests import oracle_lib as orc
 
 pytestmark = pytest.mark.gpu
@@ -294,24 +295,29 @@ def test_ensemble_member_metropolis_against_oracle(ensemble, member):
     assert np.abs(confs[member] - want).max() < 1e-10
 
 
+@pytest.mark.parametrize("whole_flow", [False, True])
 @pytest.mark.parametrize("member", [0, 9])
-def test_ensemble_member_batch_flow_against_oracle(ensemble, member):
+def test_ensemble_member_batch_flow_against_oracle(ensemble, member, whole_flow):
     """The FP64 batch flow of the whole ensemble, first 10 rows of two 8^4
     members against the oracle's flow of the same links: plaquette and <E>
-    at 1e-12 relative, Q at 1e-12 of its cancellation scale."""
+    at 1e-12 relative, Q at 1e-12 of its cancellation scale; with one launch
+    per RK stage and with one whole-flow launch per measurement interval
+    (measured every 3 steps)."""
     confs, _ = ensemble
-    eps, tau = 0.01, 0.1
+    eps, tau, every = 0.01, 0.1, 3
     with BatchContext(ENSEMBLE, "fp64") as b:
+        if whole_flow:
+            b.set_schedule(0, "column", 4096, lds=True, whole_flow=True)
         for c, L in enumerate(confs):
             b.upload(c, L)
-        series = b.flow(eps, tau, 1, use_graph=True)
+        series = b.flow(eps, tau, every, use_graph=True)
     dims = ENSEMBLE[member]
-    want, _ = orc.flow(confs[member], dims, eps, tau, 1, 0)
+    want, _ = orc.flow(confs[member], dims, eps, tau, every, 0)
     scale = orc.observables(confs[member], dims)[3]
     got = series[member]
     assert got.shape == want.shape == (10, 4)
     assert np.array_equal(got[:, 0], want[:, 0])
-    for r in range(10):
+    for r in range(0, 10, every):
         assert abs(got[r, 1] / want[r, 1] - 1) < 1e-12, r
         assert abs(got[r, 3] / want[r, 3] - 1) < 1e-12, r
         assert abs(got[r, 2] - want[r, 2]) < 1e-12 * scale, r
@@ -344,3 +350,114 @@ def test_two_row_batch_matches_single_lattice_path(inputs, grid, kind, target, l
             m = ~np.isnan(s[:, 1])
             assert np.allclose(series[c][m][:, [0, 1, 3]], s[m][:, [0, 1, 3]], rtol=1e-14, atol=0)
             assert np.allclose(series[c][m][:, 2], s[m][:, 2], rtol=0, atol=1e-13)
+
+
+# ---- one persistent launch per measurement interval ------------------------------
+
+WDIMS = [(8, 8, 8, 8), (12, 4, 6, 4), (16, 4, 4, 8), (6, 6, 6, 6)]  # slab and own-link staging
+
+
+@pytest.fixture(scope="module")
+def winputs():
+    return [orc.hot_start(d, 300 + i) for i, d in enumerate(WDIMS)]
+
+
+def _batch_flow(links, dims, precision, eps, tau, every, use_graph, **sched):
+    with BatchContext(dims, precision) as b:
+        b.set_schedule(**sched)
+        for c, L in enumerate(links):
+            b.upload(c, L)
+        series = b.flow(eps, tau, every, use_graph=use_graph)
+        return series, [b.download(c) for c in range(len(dims))], b.info()
+
+
+@pytest.mark.parametrize("precision,two_row", [("fp64", False), ("mixed", False), ("mixed", True)])
+@pytest.mark.parametrize("grid,kind,target,lds", [(0, "column", 256, True), (64, "linear", 0, True),
+                                                   (0, "contig", 0, False)])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_whole_flow_launch_matches_per_stage_launches(winputs, precision, two_row, grid, kind, target, lds,
+                                                      use_graph):
+    """The steps between two measurements as ONE persistent launch, with
+    grid-wide barriers between RK stages, give the links and the series of
+    one launch per RK stage bit for bit: 9 steps measured every 4 (three
+    launches, the last of one step), on lattices with slab and with own-link
+    staging, with the co-resident grid and with 64 workgroups (several slots
+    per workgroup and stage)."""
+    eps, tau, every = 0.02, 0.2, 4
+    common = dict(kind=kind, target=target, lds=lds, two_row=two_row)
+    s0, l0, _ = _batch_flow(winputs, WDIMS, precision, eps, tau, every, use_graph, grid=0, **common)
+    s1, l1, info = _batch_flow(winputs, WDIMS, precision, eps, tau, every, use_graph, grid=grid, whole_flow=True,
+                               **common)
+    assert info["whole_flow"] and info["grid"] > 0 and info["grid"] % 8 == 0, info
+    if grid:
+        assert info["grid"] == grid
+    for c in range(len(WDIMS)):
+        np.testing.assert_array_equal(l1[c], l0[c])
+    np.testing.assert_array_equal(np.nan_to_num(s1, nan=7.0), np.nan_to_num(s0, nan=7.0))
+
+
+def test_whole_flow_graph_replays_match_eager(winputs):
+    """Replays of the captured whole-flow graph (a barrier-state reset and a
+    persistent launch per measurement interval) with eager work between them
+    equal an eager run each time."""
+    eps, tau, every = 0.02, 0.16, 3
+    with BatchContext(WDIMS, "mixed") as b:
+        b.set_schedule(0, "column", 512, lds=True, two_row=True, whole_flow=True)
+        for c, L in enumerate(winputs):
+            b.upload(c, L)
+        eager = b.flow(eps, tau, every, use_graph=False)
+        eager_links = [b.download(c) for c in range(len(WDIMS))]
+        for rep in range(4):
+            for c, L in enumerate(winputs):
+                b.upload(c, L)
+            b.observables()  # eager work between replays
+            got = b.flow(eps, tau, every, use_graph=True)
+            np.testing.assert_array_equal(np.nan_to_num(got, nan=7.0), np.nan_to_num(eager, nan=7.0))
+            for c in range(len(WDIMS)):
+                np.testing.assert_array_equal(b.download(c), eager_links[c])
+
+
+def test_whole_flow_refuses_a_grid_beyond_the_resident_one():
+    with BatchContext(WDIMS, "mixed") as b:
+        with pytest.raises(LymError, match="resident"):
+            b.set_schedule(1 << 20, "column", 256, lds=True, two_row=True, whole_flow=True)
+        assert not b.info()["whole_flow"]  # the batch keeps its schedule
+
+
+def test_whole_flow_barrier_timeout_is_reported_and_recovers(winputs):
+    """A zero barrier timeout, set from the host, makes the first waiting
+    workgroup give up: every workgroup of the launch (and of the launches
+    after it in the flow) returns, and lym_batch_flow reports it.  With the
+    default bound restored the same batch flows as one launch per stage does."""
+    eps, tau, every = 0.02, 0.1, 2
+    s0, l0, _ = _batch_flow(winputs, WDIMS, "mixed", eps, tau, every, False, grid=0, kind="column", target=256,
+                            lds=True, two_row=True)
+    with BatchContext(WDIMS, "mixed") as b:
+        b.set_schedule(0, "column", 256, lds=True, two_row=True, whole_flow=True)
+        for c, L in enumerate(winputs):
+            b.upload(c, L)
+        b.set_barrier_timeout(0.0)
+        with pytest.raises(LymError, match="grid barrier"):
+            b.flow(eps, tau, every, use_graph=False)
+        b.set_barrier_timeout(5.0)
+        for c, L in enumerate(winputs):
+            b.upload(c, L)
+        s1 = b.flow(eps, tau, every, use_graph=True)
+        for c in range(len(WDIMS)):
+            np.testing.assert_array_equal(b.download(c), l0[c])
+        np.testing.assert_array_equal(np.nan_to_num(s1, nan=7.0), np.nan_to_num(s0, nan=7.0))
+
+
+def test_ensemble_whole_flow_matches_per_stage(ensemble):
+    """The 64-lattice ensemble, mixed, two-row form, graph-captured: 199 steps
+    measured every 10 as 20 whole-flow launches give every lattice the links
+    and series of the per-stage launches bit for bit."""
+    confs, _ = ensemble
+    eps, tau, every = 0.01, 2.0, 10
+    common = dict(kind="column", target=4096, lds=True, two_row=True)
+    s0, l0, _ = _batch_flow(confs, ENSEMBLE, "mixed", eps, tau, every, True, grid=0, **common)
+    s1, l1, info = _batch_flow(confs, ENSEMBLE, "mixed", eps, tau, every, True, grid=0, whole_flow=True, **common)
+    assert info["whole_flow"]
+    for c in range(len(ENSEMBLE)):
+        np.testing.assert_array_equal(l1[c], l0[c])
+    np.testing.assert_array_equal(np.nan_to_num(s1, nan=7.0), np.nan_to_num(s0, nan=7.0))

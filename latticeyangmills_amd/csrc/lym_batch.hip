@@ -222,24 +222,27 @@ template <typename S> int enqueue_whole(lym_batch *b, double eps, int nsteps)
     F2 *x = static_cast<F2 *>(b->W[b->cur]);
     F2 *y = static_cast<F2 *>(b->W[1 - b->cur]);
     S *acc = static_cast<S *>(b->acc);
-    lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 0);
+    lym::k_flow_sync_reset<<<1, 256, 0, b->stream>>>(b->sync, 0);
     const int lds = batch_lds_mode(b);
     const unsigned grid = unsigned(b->grid);
+    // barrier sync groups: about sqrt(grid), so that the arrivals on one
+    // counter (a group's and the count of completed groups) are few
+    const int ngroups = std::max(1, std::min({lym::kMaxSyncGroups, b->grid, int(std::lround(std::sqrt(b->grid)))}));
     if (sizeof(S) == 4 && b->form == 2)
         lym::launch_batch_flow_persistent<float, 2>(lds, grid, b->stream, reinterpret_cast<float2 *>(x),
                                                     reinterpret_cast<float2 *>(y), reinterpret_cast<float *>(acc),
                                                     b->d_desc, b->nconf, b->ntiles, dev_sched(b), eps, nsteps, b->sync,
-                                                    b->sync_limit);
+                                                    ngroups, b->sync_limit);
     else
         lym::launch_batch_flow_persistent<S, 3>(lds, grid, b->stream, x, y, acc, b->d_desc, b->nconf, b->ntiles,
-                                                dev_sched(b), eps, nsteps, b->sync, b->sync_limit);
+                                                dev_sched(b), eps, nsteps, b->sync, ngroups, b->sync_limit);
     BLAUNCHED(b, "k_batch_flow_persistent");
     if (nsteps & 1) b->cur = 1 - b->cur;
     return LYM_OK;
 }
-// Steps per whole-flow launch at most, so that the arrival count (3 per step
-// per workgroup) and the slot counters (3 per step times the XCD's slots and
-// workgroups) stay below 2^32.
+// Steps per whole-flow launch at most, so that the arrival counts (3 per step
+// per workgroup at most) and the slot counters (3 per step times a queue's
+// slots and workgroups, at most the XCD's) stay below 2^32.
 int whole_max_steps(const lym_batch *b)
 {
     const long slots = b->sched.order.empty() ? b->ntiles : long(lym::kXcds) * b->sched.per_xcd;
@@ -324,7 +327,7 @@ int enqueue_flow(lym_batch *b, double eps, int rows, int every)
     int slot = 0, rc;
     if ((rc = to_form(b, b->rows)) || (rc = observables(b, slot++))) return rc;
     if (b->sched.kind & lym::SCHED_WHOLE_FLOW) {
-        lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 1);
+        lym::k_flow_sync_reset<<<1, 256, 0, b->stream>>>(b->sync, 1);
         for (int t = 1; t < rows;) {
             const int last = std::min((t + every - 1) / every * every, rows - 1);  // last step of the segment
             if ((rc = whole(b, eps, last - t + 1))) return rc;
@@ -845,7 +848,7 @@ int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int 
                     *ms = 1e30f;
                     return apply_schedule(b, wgrids[0], kinds[0], best_target);
                 }
-                if (i == 1) lym::k_flow_sync_reset<<<1, 1, 0, b->stream>>>(b->sync, 1);
+                if (i == 1) lym::k_flow_sync_reset<<<1, 256, 0, b->stream>>>(b->sync, 1);
                 if (int rc = time_step(ms)) return rc;
                 bool aborted = false;
                 if (i == 1)

@@ -90,23 +90,40 @@ void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout,
 
 // ------------------------------------------- persistent whole-flow launch ---
 
-// State of a whole-flow launch, in device memory: the arrival count of its
-// grid barriers and one slot counter per XCD (both zeroed by
-// k_flow_sync_reset before each launch; the counters on lines of their own),
-// and a flag raised by a workgroup that gave up waiting (cleared once per
-// flow call).
+// State of a whole-flow launch, in device memory.  Every counter sits on a
+// 256-byte line of its own: atomics on one address are served one after
+// another (about 0.13 us each, measured through a one-counter barrier of
+// 1024 workgroups), so the barrier and the slot queues spread theirs.
+//   group[g], arrivals: arrival counts of the barrier's sync groups and of
+//     the groups that completed;
+//   release[g]: the number of the last barrier that group g may leave;
+//   next[q]: schedule slots handed out by slot queue q in this launch;
+//   abort: raised by a workgroup that gave up waiting.
+// k_flow_sync_reset zeroes everything but abort before each launch, and
+// abort too once per flow call.
+constexpr int kSyncLine = 64;         // unsigneds per 256-byte line
+constexpr int kMaxSyncGroups = 128;   // sync groups of a barrier at most
+constexpr int kQueuesPerXcd = 8;      // slot queues of one XCD
+constexpr int kSlotQueues = kXcds * kQueuesPerXcd;
 struct FlowSync {
     unsigned arrivals;
     int abort;
-    unsigned pad[62];
-    unsigned next[kXcds * 64];  // next[64 x]: slots of XCD x handed out so far in this launch
+    unsigned pad[kSyncLine - 2];
+    unsigned group[kMaxSyncGroups * kSyncLine];
+    unsigned release[kMaxSyncGroups * kSyncLine];
+    unsigned next[kSlotQueues * kSyncLine];
 };
 
-static __global__ void k_flow_sync_reset(FlowSync *s, int clear_abort)
+static __global__ __launch_bounds__(256) void k_flow_sync_reset(FlowSync *s, int clear_abort)
 {
-    s->arrivals = 0u;
-    for (int x = 0; x < kXcds; x++) s->next[64 * x] = 0u;
-    if (clear_abort) s->abort = 0;
+    const int i = threadIdx.x;
+    if (i == 0) s->arrivals = 0u;
+    if (i == 0 && clear_abort) s->abort = 0;
+    if (i < kMaxSyncGroups) {
+        s->group[i * kSyncLine] = 0u;
+        s->release[i * kSyncLine] = 0u;
+    }
+    if (i < kSlotQueues) s->next[i * kSyncLine] = 0u;
 }
 
 // Whether no workgroup has given up, the same answer for the whole workgroup.
@@ -120,38 +137,55 @@ LYM_DI bool flow_sync_ok(FlowSync *sync)
     return r;
 }
 
-// Grid-wide barrier of the whole-flow launch.  Every workgroup of the launch
-// is resident at once (the host sizes the grid from the kernel's occupancy),
-// so each may wait for all the others: the n-th barrier of a launch waits
-// until the arrival count reaches n * gridDim.x.  The workgroup's barrier
-// orders its waves' stores before thread 0's agent-scope release fence and
-// arrival; after the count is reached thread 0's acquire fence, then the
-// workgroup's barrier, order them before every wave's next reads, so the next
-// RK stage reads the links the other workgroups (on every XCD) wrote.  (Fences
-// in every wave instead of one per workgroup cost the 64-lattice ensemble
-// 17 %.)  The wait is bounded: past `limit` ticks of the constant-rate wall
-// clock, after kMaxSyncPolls polls, or as soon as another workgroup has given
-// up, it raises the abort flag and returns false for the whole workgroup; the
-// caller then returns, so the launch always ends.
+// Grid-wide barrier of the whole-flow launch, the n-th of the launch.  Every
+// workgroup of the launch is resident at once (the host sizes the grid from
+// the kernel's occupancy), so each may wait for all the others.  Workgroup b
+// belongs to sync group b % ngroups (ngroups <= kMaxSyncGroups, about the
+// square root of the grid): it arrives on its group's counter; the group's
+// last arrival arrives on the count of completed groups; the last group's
+// last arrival sets every group's release flag to n; each workgroup waits for
+// its own group's flag.  Memory order: the workgroup's barrier orders its
+// waves' stores before thread 0's agent-scope fence and arrival; a group's
+// last arrival fences again before it passes the group on, and the final one
+// before it releases; a waiter fences after it sees its flag, then the
+// workgroup's barrier orders that before every wave's next reads.  So the
+// next RK stage reads the links the other workgroups (on every XCD) wrote.
+// (Fences in every wave instead of one per workgroup cost the 64-lattice
+// ensemble 17 %.)  The wait is bounded: past `limit` ticks of the
+// constant-rate wall clock, after kMaxSyncPolls polls, or as soon as another
+// workgroup has given up, it raises the abort flag and returns false for the
+// whole workgroup; the caller then returns, so the launch always ends.
 constexpr unsigned kMaxSyncPolls = 1u << 26;
-LYM_DI bool grid_sync(FlowSync *sync, unsigned target, unsigned long long limit)
+LYM_DI bool grid_sync(FlowSync *sync, unsigned n, int ngroups, unsigned long long limit)
 {
     __shared__ int go;
     __syncthreads();
     if (threadIdx.x == 0) {
+        const unsigned g = blockIdx.x % unsigned(ngroups);
+        const unsigned members = (gridDim.x - g + unsigned(ngroups) - 1) / unsigned(ngroups);
+        unsigned *flag = &sync->release[g * kSyncLine];
         __threadfence();
-        __hip_atomic_fetch_add(&sync->arrivals, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(&sync->group[g * kSyncLine], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+            n * members) {
+            __threadfence();
+            if (__hip_atomic_fetch_add(&sync->arrivals, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 ==
+                n * unsigned(ngroups)) {
+                __threadfence();
+                for (int h = 0; h < ngroups; h++)
+                    __hip_atomic_store(&sync->release[h * kSyncLine], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
         const unsigned long long t0 = (unsigned long long)wall_clock64();
         int ok = 1;
         unsigned polls = 0;
-        while (__hip_atomic_load(&sync->arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < n) {
             if (__hip_atomic_load(&sync->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                 (unsigned long long)wall_clock64() - t0 > limit || ++polls > kMaxSyncPolls) {
                 __hip_atomic_store(&sync->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(8);
+            __builtin_amdgcn_s_sleep(2);
         }
         __threadfence();
         go = ok;
@@ -160,41 +194,46 @@ LYM_DI bool grid_sync(FlowSync *sync, unsigned target, unsigned long long limit)
     return go != 0;
 }
 
-// Waves per SIMD the registers of the whole-flow kernel must allow: four for
-// the FP32 slab modes (3, 4), which then need no spills (held to five, as
-// the one-slot stage kernel of mode 3 is, the loops' extra live values spill
-// 14 VGPRs in two-row form, 50 in three-row form, and the ensemble flow runs
-// 2-3 % slower); the others take what they need.
+// Waves per SIMD the registers of the whole-flow kernel must allow: five for
+// the FP32 slab modes (3, 4), as the one-slot stage kernel of mode 3 (one
+// VGPR spills, against 14-50 before the LDS reads of the stage were kept in
+// program order); the others take what they need.
 template <typename S, int LDS> constexpr int persist_min_waves()
 {
-    return sizeof(S) == 4 && (LDS == 3 || LDS == 4) ? 4 : 1;
+    return sizeof(S) == 4 && (LDS == 3 || LDS == 4) ? 5 : 1;
 }
 
-// One RK stage of the whole-flow launch: the workgroups on XCD x take the
-// schedule slots of that XCD (slot 8 e + x for entry e of its row, the slots
-// the one-launch-per-stage kernels give that XCD) one at a time from the
-// XCD's counter, so that a workgroup that finishes early takes more, as the
-// hardware dispatcher does between launches.  Every workgroup of the XCD
-// takes one value past the row per stage, so stage j of a launch starts its
-// counter at j (per_xcd + gridDim.x / 8).  Thread 0 takes the next value
-// while the workgroup works on the current slot (two LDS words alternate, so
-// one workgroup barrier per slot suffices).
+// One RK stage of the whole-flow launch: the schedule slots of XCD x (slot
+// 8 e + x for entry e of its row, the slots the one-launch-per-stage kernels
+// give that XCD) are split into kQueuesPerXcd contiguous ranges, one per slot
+// queue of the XCD, and workgroup b (on XCD x = b % 8) takes slots one at a
+// time from queue x + 8 ((b / 8) % kQueuesPerXcd), so that a workgroup that
+// finishes early takes more, as the hardware dispatcher does between
+// launches.  Every workgroup of a queue takes one value past its range per
+// stage, so stage j of a launch starts the queue's counter at
+// j (range + users).  Thread 0 takes the next value while the workgroup works
+// on the current slot (two LDS words alternate, so one workgroup barrier per
+// slot suffices).
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS>
 LYM_DI void whole_flow_stage(unsigned j, const F2<S> *win, F2<S> *wout, S *__restrict__ acc,
                              const ConfDesc *__restrict__ desc, int nconf, long slots, long per_xcd, Sched sc,
                              double eps, F2<S> *lds, FlowSync *sync, unsigned tid)
 {
     __shared__ unsigned taken[2];
-    const unsigned x = blockIdx.x % kXcds;
-    unsigned *counter = &sync->next[64 * x];
-    const unsigned base = j * unsigned(per_xcd + gridDim.x / kXcds);
+    const unsigned x = blockIdx.x % kXcds, wx = gridDim.x / kXcds;  // workgroups per XCD
+    const unsigned nq = wx < unsigned(kQueuesPerXcd) ? wx : unsigned(kQueuesPerXcd);  // queues in use
+    const unsigned qi = (blockIdx.x / kXcds) % nq;
+    const unsigned users = (wx - qi + nq - 1) / nq;                // workgroups on this queue
+    const long e0 = per_xcd * long(qi) / long(nq), e1 = per_xcd * long(qi + 1) / long(nq);  // its entries
+    unsigned *counter = &sync->next[(x + kXcds * qi) * kSyncLine];
+    const unsigned base = j * unsigned(e1 - e0 + users);
     unsigned pending = 0;
     if (threadIdx.x == 0) pending = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int i = 0;; i ^= 1) {
         if (threadIdx.x == 0) taken[i] = pending;
         __syncthreads();
-        const long e = long(taken[i]) - long(base);
-        if (e >= per_xcd) break;
+        const long e = e0 + long(taken[i]) - long(base);
+        if (e >= e1) break;
         if (threadIdx.x == 0)
             pending = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const long k = kXcds * e + x;
@@ -209,11 +248,13 @@ LYM_DI void whole_flow_stage(unsigned j, const F2<S> *win, F2<S> *wout, S *__res
 // it (the result of nsteps steps is in w0 for nsteps even, w1 for odd).  Each
 // link is computed by the same code as in the per-stage launches, so the two
 // give the same bits whichever workgroup takes a slot.  gridDim.x is a
-// multiple of 8.
+// multiple of 8; ngroups, the barrier's sync groups, is 1 .. kMaxSyncGroups
+// and at most gridDim.x.
 template <typename S, bool REUNIT, int LDS, int ROWS = 3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(persist_min_waves<S, LDS>())))
 void k_batch_flow_persistent(F2<S> *w0, F2<S> *w1, S *__restrict__ acc, const ConfDesc *__restrict__ desc, int nconf,
-                             long ntiles, Sched sc, double eps, int nsteps, FlowSync *sync, unsigned long long limit)
+                             long ntiles, Sched sc, double eps, int nsteps, FlowSync *sync, int ngroups,
+                             unsigned long long limit)
 {
     __shared__ F2<S> lds[stage_lds_elems<LDS == 4 ? 3 : LDS>()];
     const long slots = sc.order ? long(kXcds) * sc.per_xcd : ntiles;
@@ -230,11 +271,11 @@ void k_batch_flow_persistent(F2<S> *w0, F2<S> *w1, S *__restrict__ acc, const Co
     };
     for (int t = 0; t < nsteps; t++) {
         whole_flow_stage<S, 1, REUNIT, LDS, ROWS>(n, x, y, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
-        if (!grid_sync(sync, ++n * gridDim.x, limit)) return;
+        if (!grid_sync(sync, ++n, ngroups, limit)) return;
         whole_flow_stage<S, 2, REUNIT, LDS, ROWS>(n, y, x, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
-        if (!grid_sync(sync, ++n * gridDim.x, limit)) return;
+        if (!grid_sync(sync, ++n, ngroups, limit)) return;
         whole_flow_stage<S, 3, REUNIT, LDS, ROWS>(n, x, y, acc, desc, nconf, slots, per_xcd, sc, eps, lds, sync, tid());
-        if (t + 1 < nsteps && !grid_sync(sync, ++n * gridDim.x, limit)) return;
+        if (t + 1 < nsteps && !grid_sync(sync, ++n, ngroups, limit)) return;
         F2<S> *const z = x;
         x = y;
         y = z;

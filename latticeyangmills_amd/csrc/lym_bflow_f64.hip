@@ -5,7 +5,7 @@
 
 namespace lym {
 template void launch_batch_flow_persistent<double, 3>(int, unsigned, hipStream_t, F2<double> *, F2<double> *, double *,
-                                                    const ConfDesc *, int, long, Sched, double, int, FlowSync *,
+                                                    const ConfDesc *, int, long, Sched, double, int, FlowSync *, int,
                                                     unsigned long long);
 template hipError_t batch_flow_persistent_per_cu<double, 3>(int, int *);
 }  // namespace lym

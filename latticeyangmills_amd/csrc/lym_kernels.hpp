@@ -73,9 +73,11 @@ template <typename S, bool XROWS, int ROWS = 3> struct RunLinks {
     const F2<S> *lds;
     template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
-        if constexpr (ZT == 0 && DY == 0 && (DX == 0 || XROWS))
-            return lds_link<R, S>(lds + dir * 9 * kSitesPerBlock + unsigned(site - site0), kSitesPerBlock);
-        else
+        if constexpr (ZT == 0 && DY == 0 && (DX == 0 || XROWS)) {
+            unsigned i = unsigned(site - site0);
+            asm volatile("" : "+v"(i));
+            return lds_link<R, S>(lds + dir * 9 * kSitesPerBlock + i, kSitesPerBlock);
+        } else
             return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
@@ -117,7 +119,8 @@ template <typename S, int ROWS = 3> struct SlabLinks {
     {
         if constexpr (ZT == 0) {
             if (slab_staged(dir, DY)) {
-                const int li = (row - slab_row0(dir) + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
+                int li = (row - slab_row0(dir) + DY) * (lx_mask + 1) + ((x + DX) & lx_mask);
+                asm volatile("" : "+v"(li));
                 return lds_link<R, S>(slab + dir * 9 * kSlabSites + li, kSlabSites);
             }
         }
@@ -152,7 +155,7 @@ template <typename S> struct EOLinks {
 // element is a whole 128-byte line in FP32; narrower tiles (8 x 8 for
 // Lx = 24) were measured slower than staging own links only.
 constexpr int kTileX = 16, kTileY = kSitesPerBlock / kTileX;
-constexpr int kTileSites = 128;  // (kTileX + 2) * (kTileY + 2) = 108, rounded up
+constexpr int kTileSites = 108;  // (kTileX + 2) * (kTileY + 2)
 inline bool tile_fits(const int dims[4]) { return dims[0] % kTileX == 0 && dims[1] % kTileY == 0; }
 template <typename S, int ROWS = 3> struct TileLinks {
     static constexpr int rows = ROWS;
@@ -162,9 +165,11 @@ template <typename S, int ROWS = 3> struct TileLinks {
     int self;  // this lane's slot (ly + 1) * (kTileX + 2) + lx + 1
     template <typename R, int DX, int DY, int ZT, int FLIP = 0> LYM_DI su3<R> get(int dir, long site) const
     {
-        if constexpr (ZT == 0)
-            return lds_link<R, S>(tile + dir * 9 * kTileSites + self + DY * (kTileX + 2) + DX, kTileSites);
-        else
+        if constexpr (ZT == 0) {
+            int base = self;
+            asm volatile("" : "+v"(base));
+            return lds_link<R, S>(tile + dir * 9 * kTileSites + base + DY * (kTileX + 2) + DX, kTileSites);
+        } else
             return load_link_r<R, S, ROWS>(f + (long)dir * plane_elems<ROWS>() * V, V, unsigned(site));
     }
 };
@@ -305,7 +310,7 @@ template <int LDS> constexpr int stage_lds_elems()
 // Waves per SIMD the register allocation of a stage kernel must allow: the
 // FP32 slab kernel fits five workgroups' LDS on a CU, so it is held to the
 // registers of five waves; the others take what they need.
-template <typename S, int LDS> constexpr int stage_min_waves() { return sizeof(S) == 4 && LDS == 3 ? 5 : 1; }
+template <typename S, int LDS> constexpr int stage_min_waves() { return sizeof(S) == 4 && (LDS == 3 || LDS == 5) ? 5 : 1; }
 
 // tid: the thread's index in the workgroup (threadIdx.x; a persistent caller
 // passes it through an opaque copy, so that values derived from it are formed

@@ -89,16 +89,17 @@ void launch_batch_stage_r2(int lds, unsigned grid, hipStream_t st, const F2<floa
 
 // The whole-flow persistent launch over a batch (k_batch_flow_persistent):
 // nsteps RK3 steps of every lattice by grid workgroups that are all resident
-// at once; lds as launch_batch_stage, ROWS the form of the fields.
+// at once, in ngroups barrier sync groups; lds as launch_batch_stage, ROWS
+// the form of the fields.
 template <typename S, int ROWS>
 void launch_batch_flow_persistent(int lds, unsigned grid, hipStream_t st, F2<S> *w0, F2<S> *w1, S *acc,
                                   const ConfDesc *desc, int nconf, long ntiles, Sched sc, double eps, int nsteps,
-                                  FlowSync *sync, unsigned long long limit)
+                                  FlowSync *sync, int ngroups, unsigned long long limit)
 {
     constexpr bool re = sizeof(S) == 4;  // mixed path: reunitarise before the store
 #define LYM_PERSIST_CASE(L)                                                                                   \
     k_batch_flow_persistent<S, re, L, ROWS><<<grid, 256, 0, st>>>(w0, w1, acc, desc, nconf, ntiles, sc, eps, \
-                                                                   nsteps, sync, limit)
+                                                                   nsteps, sync, ngroups, limit)
     switch (lds) {
     case 0: LYM_PERSIST_CASE(0); break;
     case 1: LYM_PERSIST_CASE(1); break;
@@ -197,7 +198,7 @@ extern template void launch_batch_stage_r2<3>(int, unsigned, hipStream_t, const 
 #define LYM_BFLOW_EXTERN(S, ROWS)                                                                                   \
     extern template void launch_batch_flow_persistent<S, ROWS>(int, unsigned, hipStream_t, F2<S> *, F2<S> *, S *,   \
                                                                const ConfDesc *, int, long, Sched, double, int,     \
-                                                               FlowSync *, unsigned long long);                     \
+                                                               FlowSync *, int, unsigned long long);                \
     extern template hipError_t batch_flow_persistent_per_cu<S, ROWS>(int, int *);
 LYM_BFLOW_EXTERN(double, 3)
 LYM_BFLOW_EXTERN(float, 3)

@@ -51,11 +51,12 @@ FLOP_PER_LINK_UPDATE = {"mixed": 9245, "fp64": 9513}
 PROFILE_FILE = "profiles/k_flow_stage_32x32x32x64.md"
 PROFILE_FILE_48 = "profiles/k_flow_stage_48x48x48x96.md"
 PROFILED_TRAFFIC = {
-    ("mixed-32x64", "mixed", "column+lds+2row target=4096"): (1.441e9, PROFILE_FILE),
+    ("mixed-32x64", "mixed", "column+lds+2row target=4096"): (1.444e9, PROFILE_FILE),
     ("mixed-32x64", "mixed", "linear+lds+2row target=0"): (1.872e9, PROFILE_FILE),
-    ("mixed-32x64", "fp64", "column+lds target=4096"): (6.338e9, PROFILE_FILE),
+    ("mixed-32x64", "fp64", "column+lds target=4096"): (6.396e9, PROFILE_FILE),
+    ("mixed-48x96", "mixed", "column+lds+2row target=4096"): (8.530e9, PROFILE_FILE_48),
     ("mixed-48x96", "mixed", "column+lds+2row target=1024"): (8.736e9, PROFILE_FILE_48),
-    ("mixed-48x96", "fp64", "column+lds target=1024"): (25.54e9, PROFILE_FILE_48),
+    ("mixed-48x96", "fp64", "column+lds target=1024"): (25.81e9, PROFILE_FILE_48),
 }
 
 

@@ -1,7 +1,8 @@
 /*
  * lym_batch.h -- C ABI of the batched ensemble path: many independent
- * periodic lattices of uneven sizes, flowed by one persistent launch per RK
- * stage with the flow loop captured in a HIP graph.
+ * periodic lattices of uneven sizes, flowed together (one launch per RK stage
+ * for all of them, or one persistent launch per measurement interval), with
+ * the flow loop captured in a HIP graph.
  *
  * The reference processes its input configurations one after another
  * (WilsonFlow::flowConfigurations, src/Apps/wilsonflow.cpp:96-109, one
@@ -72,7 +73,8 @@ int lym_batch_set_schedule(lym_batch *b, int grid, int kind, int target);
 int lym_batch_set_barrier_timeout(lym_batch *b, double seconds);
 /* Times the candidate shapes on the batch itself and keeps the fastest, as
  * lym_autotune (the default schedule unless another is faster by more than
- * 1.5 %; then persistent grids of the chosen order under the same rule);
+ * 1.5 %; then persistent grids of the chosen order, then the whole-flow
+ * launch, under the same rule);
  * cache_path (may be NULL) holds one line per batch shape and precision.
  * summary receives a one-line description of the choice. */
 int lym_batch_autotune(lym_batch *b, const char *cache_path, char *summary, int summary_len);

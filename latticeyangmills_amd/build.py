@@ -34,12 +34,14 @@ LIB_SOURCES = ["lym_bflow_f64.hip", "lym_bflow_f32.hip", "lym_bflow_f32r2.hip", 
                "lym_bstage_f32.hip", "lym_bstage_f32r2.hip", "lym_stage_f64.hip", "lym_stage_f32.hip",
                "lym_stage_f32r2.hip", "lym_metro_f64.hip", "lym_metro_f32.hip", "lym_capi.hip", "lym_batch.hip",
                "lym_noalloc.hip"]
-# The whole-flow kernels loop over RK stages and schedule slots; machine-level
-# loop-invariant code motion would hoist the constants and lane offsets of the
-# stage body out of those loops and hold them in registers across them (about
-# 155 VGPRs, 3 waves per SIMD, against 96 and 5 for the one-launch-per-stage
-# kernels), so their units are built without it.
-UNIT_FLAGS = {s: ["-mllvm", "-disable-machine-licm"] for s in LIB_SOURCES if s.startswith("lym_bflow_")}
+# The batch kernels that loop over schedule slots (the persistent grid of a
+# stage, the whole-flow launch) would have machine-level loop-invariant code
+# motion hoist the constants of the stage body out of those loops and hold
+# them in registers across them (145-156 VGPRs, 3 waves per SIMD, against
+# 91-96 and 5 without it), so the batch units are built without it; the
+# one-workgroup-per-slot kernels in the same units take the same time either way.
+UNIT_FLAGS = {s: ["-mllvm", "-disable-machine-licm"] for s in LIB_SOURCES
+              if s.startswith("lym_bflow_") or s.startswith("lym_bstage_")}
 OBJ_DIR = os.path.join(PKG, "_build")
 
 

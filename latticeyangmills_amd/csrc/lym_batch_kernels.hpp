@@ -69,9 +69,11 @@ LYM_DI void batch_slot(long k, const F2<S> *__restrict__ win, F2<S> *__restrict_
 
 // PERSIST = false: one workgroup per schedule slot.  PERSIST = true: a
 // persistent grid (fewer workgroups, a multiple of 8 so that a workgroup keeps
-// its XCD) strides over the slots.  The loop keeps values live across
-// iterations and costs the mixed kernel some 50 registers (3 instead of 5
-// waves per SIMD), so the one-slot form is a kernel of its own.
+// its XCD) strides over the slots.  The one-slot form is a kernel of its own,
+// so that no loop state weighs on its registers; the units of these kernels
+// are built without machine-level loop-invariant code motion, which held the
+// stage body's constants across the grid-stride loop (145-156 VGPRs instead
+// of 93-95).
 template <typename S, int STAGE, bool REUNIT, int LDS, int ROWS = 3, bool PERSIST = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PERSIST ? 1 : stage_min_waves<S, LDS>())))
 void k_batch_flow_stage(const F2<S> *__restrict__ win, F2<S> *__restrict__ wout, S *__restrict__ acc,

@@ -389,6 +389,10 @@ def main():
         rows = [i for i in range(len(s64)) if math.isfinite(s64[i, 1])]
         e_err = max(abs(sm[i, 3] / s64[i, 3] - 1.0) for i in rows)
         q_err = max(abs(sm[i, 2] - s64[i, 2]) for i in rows)
+        p_err = max(abs(sm[i, 1] / s64[i, 1] - 1.0) for i in rows)
+        from latticeyangmills_amd import scale as _scale
+        t2e = [(s64[i, 0] ** 2 * _scale.e_mean(sm[i, 3], V), s64[i, 0] ** 2 * _scale.e_mean(s64[i, 3], V)) for i in rows]
+        t2e_err = max((abs(a / b - 1.0) for a, b in t2e if b != 0), default=None)
         from latticeyangmills_amd import scale
         t0m, t064 = scale.t0(sm, V), scale.t0(s64, V)
         w0m, w064 = scale.w0(sm, V), scale.w0(s64, V)
@@ -399,7 +403,7 @@ def main():
         out["accuracy_vs_fp64"] = {
             "flow": f"whole flow, eps {eps} to t = {s64[-1, 0]:.2f} ({len(s64) - 1} steps), measured every {every} "
                     f"steps (last measured row t = {s64[last, 0]:.2f})",
-            "max_rel_err_E": e_err, "max_abs_err_Q": q_err,
+            "max_rel_err_E": e_err, "max_rel_err_t2E": t2e_err, "max_rel_err_plaq": p_err, "max_abs_err_Q": q_err,
             "t0_a2_mixed": t0m, "t0_a2_fp64": t064,
             "t0_rel_err": (abs(t0m / t064 - 1) if (t0m and t064) else None),
             "w0_a_mixed": w0m, "w0_a_fp64": w064,

@@ -56,3 +56,52 @@ def test_create_checks_the_lattice_before_the_device():
     assert (1 << 31) // 48 < 64 * 64 * 64 * 171 and 64 * 64 * 64 * 170 < (1 << 31) // 48
     rc = lib.lym_create(ctypes.byref(ctx), (ctypes.c_int * 4)(2, 2, 2, 2), 7, 0)
     assert rc == -1  # unknown precision
+
+
+def test_null_handles_are_refused_without_a_gpu():
+    """Every entry point that takes a context or a batch refuses a null one
+    with LYM_ERR_ARG (destroy accepts it as a no-op) before it touches a
+    device, and no call throws across the ABI; checked on a machine with no
+    GPU, so a call that reached the runtime first would report another code."""
+    c = ctypes
+    lib = c.CDLL(LIB)
+    P, I, D, L = c.c_void_p, c.c_int, c.c_double, c.c_int64
+    buf = (c.c_double * 64)()
+    ptrs = (c.c_void_p * 4)(*[c.cast(buf, c.c_void_p)] * 4)
+    i4 = (c.c_int * 4)(4, 4, 4, 4)
+    n = c.c_int()
+    calls = {
+        "lym_destroy": ([P], [None], 0),
+        "lym_info": ([P, P, P, P], [None, i4, None, None], -1),
+        "lym_upload_links": ([P, P], [None, ptrs], -1),
+        "lym_download_links": ([P, P], [None, ptrs], -1),
+        "lym_set_cold": ([P], [None], -1),
+        "lym_set_hot": ([P, c.c_uint64], [None, 1], -1),
+        "lym_staples": ([P, I, P], [None, 0, buf], -1),
+        "lym_derivative": ([P, I, P], [None, 0, buf], -1),
+        "lym_observables": ([P, P], [None, buf], -1),
+        "lym_flow_steps": ([P, D, I], [None, 0.01, 1], -1),
+        "lym_flow": ([P, D, D, I, P, I, P], [None, 0.01, 0.1, 1, buf, 16, c.byref(n)], -1),
+        "lym_set_graph": ([P, I], [None, 1], -1),
+        "lym_metropolis": ([P, D, D, I, I, I, I, c.c_uint64, P, P], [None, 6.0, 0.24, 1, 1, 0, 1, 1, None, None], -1),
+        "lym_check_links": ([P, P, P], [None, buf, None], -1),
+        "lym_synchronize": ([P], [None], -1),
+        "lym_profile_flow": ([P, D, I, I, P, P], [None, 0.01, 1, 1, buf, buf], -1),
+        "lym_set_schedule": ([P, I, I], [None, 2, 4096], -1),
+        "lym_autotune": ([P, P, P, I], [None, None, None, 0], -1),
+        "lym_batch_destroy": ([P], [None], 0),
+        "lym_batch_info": ([P, P, P, P, P, P], [None, None, None, None, None, None], -1),
+        "lym_batch_upload_links": ([P, I, P], [None, 0, ptrs], -1),
+        "lym_batch_download_links": ([P, I, P], [None, 0, ptrs], -1),
+        "lym_batch_set_hot": ([P, P], [None, buf], -1),
+        "lym_batch_metropolis": ([P, D, D, I, I, I, I, P, P], [None, 6.0, 0.24, 1, 1, 0, 1, buf, None], -1),
+        "lym_batch_observables": ([P, P], [None, buf], -1),
+        "lym_batch_flow": ([P, D, D, I, I, P, I, P], [None, 0.01, 0.1, 1, 1, buf, 16, c.byref(n)], -1),
+        "lym_batch_set_schedule": ([P, I, I, I], [None, 0, 2, 4096], -1),
+        "lym_batch_set_barrier_timeout": ([P, D], [None, 1.0], -1),
+        "lym_batch_autotune": ([P, P, P, I], [None, None, None, 0], -1),
+    }
+    for name, (argtypes, args, want) in calls.items():
+        f = getattr(lib, name)
+        f.argtypes, f.restype = argtypes, I
+        assert f(*args) == want, name

@@ -461,3 +461,30 @@ def test_ensemble_whole_flow_matches_per_stage(ensemble):
     for c in range(len(ENSEMBLE)):
         np.testing.assert_array_equal(l1[c], l0[c])
     np.testing.assert_array_equal(np.nan_to_num(s1, nan=7.0), np.nan_to_num(s0, nan=7.0))
+
+
+def test_tune_cache_may_name_the_whole_flow_launch(winputs, tmp_path):
+    """A tuning cache line whose kind carries LYM_SCHED_WHOLE_FLOW (written
+    here by hand, since the tuner's own pick depends on timing) is applied as
+    it stands: the batch reports the whole-flow form with a resident grid and
+    flows bit for bit as the per-stage launches do."""
+    from latticeyangmills_amd.batch import SCHED_WHOLE_FLOW
+    from latticeyangmills_amd.capi import SCHED_LDS, SCHED_TWO_ROW, SCHEDULES
+    eps, tau, every = 0.02, 0.14, 3
+    s0, l0, _ = _batch_flow(winputs, WDIMS, "mixed", eps, tau, every, True, grid=0, kind="column", target=512,
+                            lds=True, two_row=True)
+    key = "gfx950|prec1|" + "|".join("x".join(map(str, d)) for d in WDIMS)
+    kind = SCHEDULES["column"] | SCHED_LDS | SCHED_TWO_ROW | SCHED_WHOLE_FLOW
+    cache = tmp_path / "tune.txt"
+    cache.write_text(f"{key}\t0 {kind} 512 1.0\n")
+    with BatchContext(WDIMS, "mixed") as b:
+        assert "cached" in b.autotune(str(cache))
+        info = b.info()
+        assert info["whole_flow"] and info["two_row"] and info["lds"] and info["kind"] == "column", info
+        assert info["grid"] > 0 and info["grid"] % 8 == 0
+        for c, L in enumerate(winputs):
+            b.upload(c, L)
+        s1 = b.flow(eps, tau, every, use_graph=True)
+        for c in range(len(WDIMS)):
+            np.testing.assert_array_equal(b.download(c), l0[c])
+    np.testing.assert_array_equal(np.nan_to_num(s1, nan=7.0), np.nan_to_num(s0, nan=7.0))
